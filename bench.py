@@ -1,0 +1,190 @@
+"""Benchmark of the fused D3Q19 wind-tunnel step on one MI355X.
+
+Workload (the single-GPU configuration the headline metric is quoted on):
+a 512 x 256 x 256 = 33.5M-cell tunnel, car-like body = seeded union of 12
+ellipsoids on a moving ground belt, uin 0.05, tau 0.51, Cs 0.16, 2^20 particles
+seeded on the inlet plane and advected every step, three mid-span slice planes
+(speed, pressure, vorticity) and the force extracted every 10 steps. A "step"
+is one lattice update of the whole tunnel plus the particle advection.
+
+value = MLUPS with FP32 populations (all nx*ny*nz cells per step, as
+server/app.py:180 counts them); the same workload with FP16-deviation storage
+is reported beside it. roofline: algorithmic bytes of the step kernel
+(169 B/cell FP32, 93 B/cell FP16: 19 populations read and written, the class
+byte, the 16 B macroscopic store) over its mean launch time, from HIP events on
+the stream the kernel runs on. cpu_baseline: the CPU restatement of the same
+step (oracle/, C++ with OpenMP) timed on this host on a bounded sample.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+REPO = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(REPO, 'wind-tunnel_amd'))
+sys.path.insert(0, REPO)
+
+PEAK_HBM_GBS = 8000.0
+BYTES_PER_CELL = {'fp32': 169, 'fp16': 93, 'bf16': 93, 'fp64': 321}
+DIMS = (512, 256, 256)
+UIN, TAU, SMAG = 0.05, 0.51, 0.16
+BODY_SEED = 2025
+PARTICLES = 1 << 20
+
+
+def log(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
+def build_mask(dims):
+    from windtunnel import geometry
+    ells = geometry.car_body(dims, BODY_SEED)
+    mask, counts = geometry.ellipsoids(ells, dims, ground=True)
+    lref = float(ells[:, 0].max() + ells[:, 3].max() - (ells[:, 0].min() - ells[:, 3].max()))
+    return mask, counts, lref
+
+
+def run_gpu(form, steps, warmup, mask, counts, lref):
+    import torch
+    from windtunnel.session import Session
+    nx, ny, nz = DIMS
+    slices = [(2, nz // 2, 0), (2, nz // 2, 1), (2, nz // 2, 2)]
+    s = Session(DIMS, mask, form, UIN, TAU, SMAG, particles=PARTICLES, slices=slices, frame_every=10,
+                lref=lref, seed=7, frontal=counts['frontal'])
+    stream = torch.cuda.current_stream()
+    for _ in range(warmup):
+        s.step()
+    torch.cuda.synchronize()
+    ev = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in range(steps)]
+    is_force = []
+    t_start = torch.cuda.Event(enable_timing=True)
+    t_end = torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize()
+    h0 = time.perf_counter()
+    t_start.record(stream)
+    for k in range(steps):
+        is_force.append(s.is_frame())
+        ev[k][0].record(stream)
+        s.T.step(1, s.uin, s.tau, s.smag, flags=1 if s.is_frame() else 0)
+        ev[k][1].record(stream)
+        if s.num_p:
+            s.seed += 1
+            s.T.advect(s.pos, s.spd, 1.0, s.uin, s.max_age, s.seed, s.box)
+        if s.is_frame():
+            s.emit_frame()
+        s.step_count += 1
+    t_end.record(stream)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - h0
+    gpu_ms = t_start.elapsed_time(t_end)
+    plain = [ev[k][0].elapsed_time(ev[k][1]) for k in range(steps) if not is_force[k]]
+    forced = [ev[k][0].elapsed_time(ev[k][1]) for k in range(steps) if is_force[k]]
+    step_ms = sum(plain) / max(1, len(plain))
+    cd, cl = s.coefficients()
+    n = s.n
+    res = {
+        'form': form, 'wall_s': wall, 'gpu_ms': gpu_ms, 'mlups': n * steps / wall / 1e6,
+        'ms_per_step': wall * 1e3 / steps, 'step_kernel_ms': step_ms,
+        'step_kernel_force_ms': sum(forced) / max(1, len(forced)),
+        'kernel_mlups': n / (step_ms * 1e-3) / 1e6,
+        'achieved_gbs': BYTES_PER_CELL[form] * n / (step_ms * 1e-3) / 1e9,
+        'cd': float(cd), 'cl': float(cl),
+    }
+    del s
+    torch.cuda.empty_cache()
+    return res
+
+
+def cpu_baseline(mask, target_s=12.0):
+    """CPU restatement of the same FP32 step (oracle, C++/OpenMP) on this host."""
+    from oracle import oracle as O
+    threads = int(os.environ.get('OMP_NUM_THREADS') or os.cpu_count() or 1)
+    O.set_threads(threads)
+    t = O.OracleTunnel(DIMS, O.FP32)
+    t.init(UIN)
+    t.set_solid(mask)
+    t0 = time.perf_counter()
+    t.step(1, UIN, TAU, SMAG)
+    one = time.perf_counter() - t0
+    k = max(1, min(50, int(target_s / max(one, 1e-3))))
+    t0 = time.perf_counter()
+    t.step(k, UIN, TAU, SMAG)
+    dt = time.perf_counter() - t0
+    n = t.n
+    return {'value': round(n * k / dt / 1e6, 2), 'unit': 'MLUPS', 'cores': threads, 'kind': 'port',
+            'sample': f'{k} FP32 lattice steps (no particles/slices) of the same 512x256x256 tunnel and '
+                      f'body, after 1 untimed step; oracle/wt_oracle.cpp with {threads} OpenMP threads'}
+
+
+def load_traffic(form):
+    path = os.path.join(REPO, 'profiles', f'step_{form}_512x256x256_traffic.json')
+    if os.path.exists(path):
+        with open(path) as fh:
+            return json.load(fh).get('hbm_bytes_per_launch')
+    return None
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument('--gpus', type=int, default=1)
+    ap.add_argument('--steps', type=int, default=1000)
+    ap.add_argument('--warmup', type=int, default=20)
+    ap.add_argument('--forms', default='fp32,fp16')
+    ap.add_argument('--no-cpu', action='store_true')
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f'bench.py measures one GPU: this build has no multi-GPU path (asked for --gpus {args.gpus}). '
+              'Independent tunnels shard one per GPU with no communication; see DESIGN.md.', file=sys.stderr)
+        sys.exit(2)
+    import torch
+    if not torch.cuda.is_available():
+        print('bench.py needs a HIP GPU', file=sys.stderr)
+        sys.exit(1)
+    from windtunnel import _lib
+    _lib.load(build_if_missing=True)
+    log(f'building geometry {DIMS} ...')
+    mask, counts, lref = build_mask(DIMS)
+    log(f'body: {counts}')
+    results = {}
+    for form in args.forms.split(','):
+        log(f'GPU run {form}: warmup {args.warmup}, {args.steps} steps ...')
+        results[form] = run_gpu(form, args.steps, args.warmup, mask, counts, lref)
+        log(json.dumps(results[form]))
+    cpu = None
+    if not args.no_cpu:
+        log('CPU baseline ...')
+        cpu = cpu_baseline(mask)
+        log(json.dumps(cpu))
+    main_form = 'fp32' if 'fp32' in results else next(iter(results))
+    r = results[main_form]
+    line = {
+        'metric': 'MLUPS @33M cells fp32 vs fp16-store; p50/p99 step latency, mixed-size serving',
+        'value': round(r['mlups'], 1), 'unit': 'MLUPS', 'n_gpus': 1, 'steps': args.steps, 'warmup': args.warmup,
+        'ms_per_step': round(r['ms_per_step'], 4), 'higher_is_better': True, 'scaling': 'weak',
+        'vs_baseline': None, 'dtype': 'f32', 'data': 'synthetic',
+        'config': {'workload': 'single tunnel 512x256x256 (33.5M cells), 12-ellipsoid car body on a moving '
+                               'belt, 2^20 particles advected every step, 3 slices + forces every 10 steps',
+                   'cells': DIMS[0] * DIMS[1] * DIMS[2], 'storage': main_form, 'uin': UIN, 'tau': TAU,
+                   'smag': SMAG, 'particles': PARTICLES, 'parallelism': 'single GPU'},
+        'roofline': {'bound': 'hbm', 'achieved': round(r['achieved_gbs'], 1), 'peak': PEAK_HBM_GBS,
+                     'unit': 'GB/s', 'frac': round(r['achieved_gbs'] / PEAK_HBM_GBS, 4),
+                     'traffic': load_traffic(main_form),
+                     'kernel': 'k_step (fused stream/collide)', 'bytes_per_cell': BYTES_PER_CELL[main_form],
+                     'kernel_ms': round(r['step_kernel_ms'], 4)},
+        'cpu_baseline': cpu,
+    }
+    for form, rr in results.items():
+        if form == main_form:
+            continue
+        line[form] = {'value': round(rr['mlups'], 1), 'ratio_vs_fp32': round(rr['mlups'] / r['mlups'], 3),
+                      'kernel_ms': round(rr['step_kernel_ms'], 4),
+                      'roofline_frac': round(rr['achieved_gbs'] / PEAK_HBM_GBS, 4),
+                      'achieved_gbs': round(rr['achieved_gbs'], 1), 'bytes_per_cell': BYTES_PER_CELL[form],
+                      'traffic': load_traffic(form)}
+    line['cd_cl'] = {f: [round(v['cd'], 4), round(v['cl'], 4)] for f, v in results.items()}
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == '__main__':
+    main()

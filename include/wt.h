@@ -1,0 +1,178 @@
+/* wt.h -- C ABI of the MI355X (gfx950) D3Q19 wind-tunnel solver library, libwt.so.
+ *
+ * Plain C: every entry point is extern "C", takes plain pointers and sizes,
+ * never throws, and returns an int status (0 = ok, < 0 = error; the message is
+ * in wt_last_error(), which is thread-local). Device buffers are allocated by
+ * the caller (e.g. torch.empty(..., device="cuda")) and passed as raw device
+ * pointers; `stream` is a hipStream_t passed as void* (NULL = default stream).
+ * All launches are asynchronous on `stream` unless stated otherwise.
+ *
+ * Reference interfaces each entry point replaces (RICKxxROLLING/wind-tunnel):
+ *   lbm.py = server/lbm.py, app.py = server/app.py, voxelize.js = web/js/voxelize.js
+ */
+#ifndef WT_H
+#define WT_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Population storage forms. FP32 stores f_i as in lbm.py:404. FP16 and BF16
+ * store the deviation g_i = f_i - w_i from the lattice weight (collision math
+ * stays FP32). FP64 stores f_i in double and does all arithmetic in double. */
+#define WT_FP32 0
+#define WT_FP16 1
+#define WT_BF16 2
+#define WT_FP64 3
+
+/* wt_step / wt_batch_step flags */
+#define WT_FLAG_FORCES 1 /* fuse the momentum-exchange force into the last step */
+#define WT_FLAG_MASS 2   /* fuse the FP64 mass reduction into the last step */
+
+/* One tunnel's device state. n = nx*ny*nz; sizes come from wt_buffer_sizes().
+ * f[0], f[1]: ping-pong population planes, SoA: population i of cell
+ *             idx = x + nx*(y + ny*z) at element i*n + idx (lbm.py:68, 118).
+ * cls:        per-cell class byte built by wt_set_solid from the solid mask.
+ * mac:        float4 (ux, uy, uz, rho) per cell (lbm.py:406).
+ * shell[0,1]: ping-pong snapshot of the macroscopic field on the six layers
+ *             next to the domain faces; face cells read the previous step's
+ *             values from it.
+ * partials:   per-tile FP64 reduction slots (force x,y,z and mass).
+ * sums:       4 doubles on the device: fx, fy, fz, mass. */
+typedef struct wt_grid {
+  int32_t nx, ny, nz;
+  int32_t form;
+  void* f[2];
+  uint8_t* cls;
+  float* mac;
+  void* shell[2];
+  double* partials;
+  double* sums;
+} wt_grid;
+
+/* Per-tunnel physical parameters, p['uIn'], p['tau'], p['smag'] (lbm.py:446). */
+typedef struct wt_params {
+  float uin;
+  float tau;
+  float smag;
+} wt_params;
+
+/* Respawn box of the particle rake: p['seedX'], ['seedCy'], ['seedCz'],
+ * ['seedHy'], ['seedHz'] (lbm.py:452-453). */
+typedef struct wt_seed_box {
+  float x, cy, cz, hy, hz;
+} wt_seed_box;
+
+/* ---- library ------------------------------------------------------------- */
+const char* wt_last_error(void);
+int wt_version(void);
+/* Device name and total memory (replaces Warp's device query, app.py:56-59). */
+int wt_device_info(char* name, int name_len, int64_t* total_bytes);
+
+/* ---- sizes and admission -------------------------------------------------- */
+/* Bytes of each buffer of wt_grid, in the order f (one buffer), cls, mac,
+ * shell (one buffer), partials, sums (sizes[6]). Replaces the per-cell
+ * footprint constant BYTES_PER_CELL (lbm.py:30) with exact sizes. */
+int wt_buffer_sizes(int nx, int ny, int nz, int form, int64_t sizes[6]);
+/* Total device bytes of one tunnel (the admission footprint, app.py:72-76). */
+int64_t wt_state_bytes(int nx, int ny, int nz, int form);
+/* Number of reduction tiles of a tunnel (partials holds 4 doubles each). */
+int64_t wt_tile_count(int nx, int ny, int nz);
+
+/* ---- state ---------------------------------------------------------------- */
+/* Equilibrium at rho = 1, u = (uin, 0, 0) in both population buffers, mac and
+ * both shells. Replaces Solver.reset / init_kernel (lbm.py:415-417, 64-69). */
+int wt_init(const wt_grid* g, float uin, void* stream);
+/* Build the class bytes from a device solid mask of n bytes (0 fluid,
+ * 1 static solid, 2 moving belt, other non-zero = solid). Replaces
+ * Solver.set_solid (lbm.py:419-421). */
+int wt_set_solid(const wt_grid* g, const uint8_t* solid_dev, void* stream);
+
+/* ---- the hot path --------------------------------------------------------- */
+/* nsteps fused D3Q19 steps starting from f[cur] (reads f[cur], writes
+ * f[cur^1], and so on). The caller flips its own cur nsteps times. flags apply
+ * to the last step; with WT_FLAG_FORCES or WT_FLAG_MASS the reduced values are
+ * in g->sums when the stream reaches the end of this call. Replaces the step
+ * loop of Solver.step (lbm.py:442-447) over step_kernel (lbm.py:72-166). */
+int wt_step(const wt_grid* g, int cur, int nsteps, const wt_params* p, int flags, void* stream);
+/* Momentum-exchange force on f[cur] into g->sums[0..2] (FP64, deterministic).
+ * Replaces forces.zero_() + force_kernel (lbm.py:479-481, 169-197). */
+int wt_forces(const wt_grid* g, int cur, void* stream);
+/* Copy g->sums to host (synchronizes the stream). */
+int wt_read_sums(const wt_grid* g, double out[4], void* stream);
+
+/* ---- batched tunnels ------------------------------------------------------ */
+/* Many resident tunnels advance one step each in ONE launch. Replaces running
+ * one Solver.step per tunnel (lbm.py:442-447) for the serving loop
+ * (app.py:150-202 generalised to many sessions). */
+typedef struct wt_batch_info {
+  int64_t total_tiles;
+  int32_t max_chunks;
+  int32_t form;
+  int32_t ntunnels;
+  int32_t cpt;
+  int32_t remap;
+  int32_t reserved;
+} wt_batch_info;
+/* Device workspace bytes for a batch of ntunnels (descriptor tables). */
+int64_t wt_batch_workspace_bytes(int ntunnels);
+/* Upload the descriptor table of a batch: grids[k] with params[k], current
+ * buffer cur[k]. All tunnels share one storage form. Synchronizes `stream`. */
+int wt_batch_prepare(const wt_grid* grids, const wt_params* params, const int* cur, int ntunnels,
+                     void* workspace, wt_batch_info* info, void* stream);
+/* One step of every tunnel of the prepared batch; with parity p tunnel k
+ * reads buffer cur[k]^p. flags as for wt_step; sums land in each grid's sums.
+ * Kernel launches only: a caller may capture it in a graph. */
+int wt_batch_step(void* workspace, const wt_batch_info* info, int parity, int flags, void* stream);
+
+/* Launch shape of the step kernels of one storage form: cells per thread
+ * (1, 2 or 4) and XCD-contiguous tile order (0/1). Chosen by the autotuner;
+ * results do not depend on it. Applies to later wt_step / wt_batch_prepare. */
+int wt_set_launch_shape(int form, int cpt, int remap);
+int wt_get_launch_shape(int form, int* cpt, int* remap);
+
+/* ---- particles, slices, surface pressure ---------------------------------- */
+/* RK2 particle advection with respawn (advect_kernel, lbm.py:233-263).
+ * pos: float4 (x, y, z, age) per particle; spd: float per particle. */
+int wt_advect(const wt_grid* g, float* pos, float* spd, int64_t num_p, float dt, float uin,
+              float max_age, uint32_t seed, const wt_seed_box* box, void* stream);
+/* Quantize particles (pack_particles, lbm.py:266-275). */
+int wt_pack_particles(const wt_grid* g, const float* pos, const float* spd, uint16_t* q16,
+                      uint8_t* q8, int64_t num_p, void* stream);
+/* One axis-aligned slice as (value byte, fluid byte) pairs (slice_kernel,
+ * lbm.py:278-318; Solver.frame lbm.py:464-474). index must be in range. */
+int wt_slice(const wt_grid* g, uint8_t* out, int axis, int index, int field, float uin, float lref,
+             float vmin, float vmax, void* stream);
+/* Surface pressure coefficient per vertex as int8 (cp_kernel, lbm.py:353-368). */
+int wt_cp(const wt_grid* g, const float* verts, const float* norms, int8_t* out, int64_t nv,
+          float uin, float cp_range, void* stream);
+
+/* ---- checks ----------------------------------------------------------------- */
+/* Per-cell link classification as the step kernel sees it (u64 per cell):
+ * bits 0-1 cell class (0 interior fluid, 1 solid, 2 face); bit 8+i bounce-back
+ * of population i; bit 27+i moving-belt term; bit 45+i link i counts toward the
+ * force (i = 1..18). Mirrors lbm.py:81-84, 114-124, 185-188. */
+int wt_link_classes(const wt_grid* g, uint64_t* out_dev, void* stream);
+
+/* ---- geometry (host code) ------------------------------------------------- */
+/* Voxelize a triangle soup given in lattice coordinates (9 floats per
+ * triangle) into out_host[n] exactly as voxelize.js:5-115 does: ray parity
+ * along +x, surface shell, flood-fill closure, ground layer y=0 tagged 2.
+ * counts = {solid cells, frontal area, planform area} (before the ground tag). */
+int wt_voxelize(const float* tris, int64_t ntris, int nx, int ny, int nz, int ground,
+                uint8_t* out_host, int64_t counts[3]);
+/* Union of ellipsoids (k rows of cx, cy, cz, ax, ay, az; a cell centre is
+ * inside if sum(((p - c)/a)^2) <= 1), clipped to [1, dim-2], closed with the
+ * voxelize.js flood fill, ground layer y=0 tagged 2 when ground != 0.
+ * Host-side geometry for analytic bodies. */
+int wt_ellipsoids(const double* ell, int k, int nx, int ny, int nz, int ground, uint8_t* out_host,
+                  int64_t counts[3]);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* WT_H */

@@ -1,0 +1,557 @@
+// CPU oracle for the D3Q19 wind-tunnel solver path.
+//
+// TEST INFRASTRUCTURE ONLY. This file is the checker: it is built into
+// oracle/liboracle.so and may be loaded only by tests/, by the smoke check in
+// __graft_entry__.py and by the cpu_baseline leg of bench.py. The product
+// library (wind-tunnel_amd/csrc) never links, loads or calls it.
+//
+// It restates the reference's algorithm step by step, in the reference's own
+// order of floating-point operations, compiled with -ffp-contract=off so every
+// +,-,*,/ and sqrt is one IEEE-754 operation:
+//   * lattice tables .......................... server/lbm.py:15-19
+//   * init (equilibrium at rho=1, u=(uin,0,0)) . server/lbm.py:64-69
+//   * fused step (faces, pull streaming, bounce-back, moving belt, moments,
+//     velocity clamp, regularized collision, Smagorinsky LES, outlet sponge)
+//                                               server/lbm.py:72-166
+//   * momentum-exchange force ................. server/lbm.py:169-197
+//   * particle advection (RK2 midpoint) ....... server/lbm.py:201-263
+//   * particle quantization ................... server/lbm.py:266-275
+//   * slice extraction ........................ server/lbm.py:278-318
+//   * surface pressure (Cp) ................... server/lbm.py:321-368
+//
+// Deliberate, documented choices where the reference is nondeterministic:
+//   * Face cells read the previous step's macroscopic field (Jacobi rule). In
+//     the reference a face reads mac[] of its nearest interior cell in the same
+//     launch that overwrites it (lbm.py:87 vs lbm.py:166), a race.
+//   * The force is accumulated in FP64 (the reference adds FP32 atomics in
+//     nondeterministic order, lbm.py:195-197). Each per-link term 2(f_i - w_i)
+//     is still formed in the population precision, as lbm.py:189 does.
+//   * The respawn random stream is this project's own counter-based hash
+//     (Warp's rand_init/randf stream, lbm.py:256-259, is not available here).
+//
+// Reduced-precision storage forms (FP16 / BF16 deviations g_i = f_i - w_i,
+// collision in FP32) are restated with software round-to-nearest-even
+// conversions so they can be compared bit for bit.
+#include <omp.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+#include <vector>
+
+namespace {
+
+const int CX[19] = {0, 1, -1, 0, 0, 0, 0, 1, -1, 1, -1, 1, -1, 1, -1, 0, 0, 0, 0};
+const int CY[19] = {0, 0, 0, 1, -1, 0, 0, 1, -1, -1, 1, 0, 0, 0, 0, 1, -1, 1, -1};
+const int CZ[19] = {0, 0, 0, 0, 0, 1, -1, 0, 0, 0, 0, 1, -1, -1, 1, 1, -1, -1, 1};
+const int OPP[19] = {0, 2, 1, 4, 3, 6, 5, 8, 7, 10, 9, 12, 11, 14, 13, 16, 15, 18, 17};
+
+// lbm.py:18 -- w = 1/3 (rest), 1/18 (axis), 1/36 (diagonal). The reference
+// prints them with repr(float) and Warp reads the literal as float32.
+inline double w_d(int i) { return i == 0 ? 1.0 / 3.0 : (i <= 6 ? 1.0 / 18.0 : 1.0 / 36.0); }
+
+template <typename T> struct Lit {
+  static T w(int i) { return (T)w_d(i); }
+  static T c(double v) { return (T)v; }
+};
+
+// ---- software conversions (round to nearest even) --------------------------
+inline uint32_t fbits(float f) { uint32_t u; std::memcpy(&u, &f, 4); return u; }
+inline float bitsf(uint32_t u) { float f; std::memcpy(&f, &u, 4); return f; }
+
+uint16_t f2h(float f) {
+  uint32_t x = fbits(f);
+  uint32_t sign = (x >> 16) & 0x8000u;
+  uint32_t ax = x & 0x7fffffffu;
+  if (ax >= 0x7f800000u) return (uint16_t)(sign | (ax > 0x7f800000u ? 0x7e00u : 0x7c00u));
+  if (ax >= 0x477ff000u) return (uint16_t)(sign | 0x7c00u);  // rounds to >= 65520 -> inf
+  if (ax < 0x38800000u) {                                    // result is subnormal or zero
+    // value = m * 2^-24 with m rounded to nearest even
+    float a = bitsf(ax);
+    double m = (double)a * 16777216.0;  // exact
+    double r = std::nearbyint(m);       // default rounding mode: nearest even
+    return (uint16_t)(sign | (uint32_t)r);
+  }
+  uint32_t mant = ax & 0x7fffffu;
+  int32_t e = (int32_t)(ax >> 23) - 127 + 15;
+  uint32_t h = ((uint32_t)e << 10) | (mant >> 13);
+  uint32_t rem = mant & 0x1fffu;
+  if (rem > 0x1000u || (rem == 0x1000u && (h & 1u))) h += 1;
+  return (uint16_t)(sign | h);
+}
+
+float h2f(uint16_t h) {
+  uint32_t sign = (uint32_t)(h & 0x8000u) << 16;
+  uint32_t e = (h >> 10) & 0x1fu, m = h & 0x3ffu;
+  if (e == 0) {
+    float v = (float)m * (1.0f / 16777216.0f);
+    return sign ? -v : v;
+  }
+  if (e == 31) return bitsf(sign | 0x7f800000u | (m << 13));
+  return bitsf(sign | ((e - 15 + 127) << 23) | (m << 13));
+}
+
+uint16_t f2bf(float f) {
+  uint32_t x = fbits(f);
+  if ((x & 0x7fffffffu) > 0x7f800000u) return (uint16_t)((x >> 16) | 0x40u);
+  x += 0x7fffu + ((x >> 16) & 1u);
+  return (uint16_t)(x >> 16);
+}
+float bf2f(uint16_t b) { return bitsf((uint32_t)b << 16); }
+
+// ---- storage forms ----------------------------------------------------------
+// FORM 0: FP32 absolute f_i; 1: FP16 deviation; 2: BF16 deviation; 3: FP64 absolute.
+template <int FORM> struct Form;
+template <> struct Form<0> {
+  typedef float elem; typedef float real;
+  static float load(elem e, int) { return e; }
+  static elem store(float f, int) { return f; }
+};
+template <> struct Form<1> {
+  typedef uint16_t elem; typedef float real;
+  static float load(elem e, int i) { return Lit<float>::w(i) + h2f(e); }
+  static elem store(float f, int i) { return f2h(f - Lit<float>::w(i)); }
+};
+template <> struct Form<2> {
+  typedef uint16_t elem; typedef float real;
+  static float load(elem e, int i) { return Lit<float>::w(i) + bf2f(e); }
+  static elem store(float f, int i) { return f2bf(f - Lit<float>::w(i)); }
+};
+template <> struct Form<3> {
+  typedef double elem; typedef double real;
+  static double load(elem e, int) { return e; }
+  static elem store(double f, int) { return f; }
+};
+
+template <typename T> T clampT(T v, T a, T b) { return std::min(std::max(v, a), b); }
+
+// lbm.py:64-69
+template <int FORM>
+void init_t(void* fv, void* macv, int64_t n, float uin_f) {
+  typedef Form<FORM> S; typedef typename S::real R;
+  typename S::elem* f = (typename S::elem*)fv;
+  R* mac = (R*)macv;
+  R uin = (R)uin_f;
+  for (int i = 0; i < 19; i++) {
+    R v = Lit<R>::w(i) * ((R)1.0 + (R)(3 * CX[i]) * uin + (R)(4.5 * CX[i] * CX[i]) * uin * uin -
+                          (R)1.5 * uin * uin);
+    typename S::elem e = S::store(v, i);
+    for (int64_t k = 0; k < n; k++) f[i * n + k] = e;
+  }
+  for (int64_t k = 0; k < n; k++) {
+    mac[4 * k] = uin; mac[4 * k + 1] = 0; mac[4 * k + 2] = 0; mac[4 * k + 3] = 1;
+  }
+}
+
+// One step of lbm.py:72-166. mac_prev is read by face cells (Jacobi rule);
+// mac_out receives this step's macroscopic field. Solid cells leave fout
+// untouched, exactly as the reference does.
+template <int FORM>
+void step_t(const void* finv, void* foutv, const uint8_t* solid, const void* mac_prevv,
+            void* mac_outv, int nx, int ny, int nz, float uin_f, float tau_f, float smag_f,
+            double* mass_out) {
+  typedef Form<FORM> S; typedef typename S::real R; typedef typename S::elem E;
+  const E* fin = (const E*)finv;
+  E* fout = (E*)foutv;
+  const R* mac_prev = (const R*)mac_prevv;
+  R* mac_out = (R*)mac_outv;
+  const int64_t n = (int64_t)nx * ny * nz;
+  const R uin = (R)uin_f, tau = (R)tau_f, smag = (R)smag_f;
+  const R zero = 0, one = 1;
+  std::vector<double> zmass(nz, 0.0);
+#pragma omp parallel for schedule(static)
+  for (int z = 0; z < nz; z++) {
+    double mass = 0.0;
+    for (int y = 0; y < ny; y++)
+      for (int x = 0; x < nx; x++) {
+        const int64_t idx = x + (int64_t)nx * (y + (int64_t)ny * z);
+        R* mo = mac_out + 4 * idx;
+        if (solid[idx] != 0) {  // lbm.py:81-83
+          mo[0] = 0; mo[1] = 0; mo[2] = 0; mo[3] = 1;
+          continue;
+        }
+        if (x == 0 || x == nx - 1 || y == 0 || y == ny - 1 || z == 0 || z == nz - 1) {
+          // lbm.py:84-109
+          int cx = clampT(x, 1, nx - 2), cy = clampT(y, 1, ny - 2), cz = clampT(z, 1, nz - 2);
+          const R* m = mac_prev + 4 * (cx + (int64_t)nx * (cy + (int64_t)ny * cz));
+          R br = clampT(m[3], (R)0.9, (R)1.1);
+          R bux = m[0], buy = m[1], buz = m[2];
+          if (y == 0 || y == ny - 1) buy = 0;
+          if (z == 0 || z == nz - 1) buz = 0;
+          if (x == 0) { br = one; bux = uin; buy = 0; buz = 0; }
+          else if (x == nx - 1) { br = one; bux = std::max(bux, zero); }
+          R bsq = (R)1.5 * (bux * bux + buy * buy + buz * buz);
+          for (int i = 0; i < 19; i++) {
+            R s = zero;
+            if (CX[i]) s = CX[i] > 0 ? s + bux : s - bux;
+            if (CY[i]) s = CY[i] > 0 ? s + buy : s - buy;
+            if (CZ[i]) s = CZ[i] > 0 ? s + buz : s - buz;
+            R bc = (R)3.0 * s;
+            R v = Lit<R>::w(i) * br * (one + bc + (R)0.5 * bc * bc - bsq);
+            fout[i * n + idx] = S::store(v, i);
+          }
+          mo[0] = bux; mo[1] = buy; mo[2] = buz; mo[3] = br;
+          mass += (double)br;
+          continue;
+        }
+        // interior fluid: pull streaming with bounce-back and belt, lbm.py:110-124
+        R f[19];
+        f[0] = S::load(fin[idx], 0);
+        for (int i = 1; i < 19; i++) {
+          int64_t s = (x - CX[i]) + (int64_t)nx * ((y - CY[i]) + (int64_t)ny * (z - CZ[i]));
+          int t = solid[s];
+          if (t == 0) {
+            f[i] = S::load(fin[i * n + s], i);
+          } else {
+            f[i] = S::load(fin[OPP[i] * n + idx], OPP[i]);
+            if (CX[i] && t == 2) f[i] = f[i] + (R)(6.0 * w_d(i) * CX[i]) * uin;
+          }
+        }
+        // moments, lbm.py:125-136 (signed sums in index order, as lin() emits them)
+        R rho = zero;
+        for (int i = 0; i < 19; i++) rho = rho + f[i];
+        R ir = one / rho;
+        R sx = zero, sy = zero, sz = zero;
+        for (int i = 0; i < 19; i++) {
+          if (CX[i] > 0) sx = sx + f[i]; else if (CX[i] < 0) sx = sx - f[i];
+        }
+        for (int i = 0; i < 19; i++) {
+          if (CY[i] > 0) sy = sy + f[i]; else if (CY[i] < 0) sy = sy - f[i];
+        }
+        for (int i = 0; i < 19; i++) {
+          if (CZ[i] > 0) sz = sz + f[i]; else if (CZ[i] < 0) sz = sz - f[i];
+        }
+        R ux = sx * ir, uy = sy * ir, uz = sz * ir;
+        R um = std::sqrt(ux * ux + uy * uy + uz * uz);
+        if (um > (R)0.35) {
+          R sc = (R)0.35 / um;
+          ux = ux * sc; uy = uy * sc; uz = uz * sc;
+        }
+        R usq = (R)1.5 * (ux * ux + uy * uy + uz * uz);
+        R e[19], d[19];
+        for (int i = 0; i < 19; i++) {
+          R s = zero;
+          if (CX[i]) s = CX[i] > 0 ? s + ux : s - ux;
+          if (CY[i]) s = CY[i] > 0 ? s + uy : s - uy;
+          if (CZ[i]) s = CZ[i] > 0 ? s + uz : s - uz;
+          R cu = (R)3.0 * s;
+          e[i] = Lit<R>::w(i) * rho * (one + cu + (R)0.5 * cu * cu - usq);
+          d[i] = f[i] - e[i];
+        }
+        // second moments of f_neq, lbm.py:141-146
+        R pxx = zero, pyy = zero, pzz = zero, pxy = zero, pxz = zero, pyz = zero;
+        for (int i = 0; i < 19; i++) if (CX[i] != 0) pxx = pxx + d[i];
+        for (int i = 0; i < 19; i++) if (CY[i] != 0) pyy = pyy + d[i];
+        for (int i = 0; i < 19; i++) if (CZ[i] != 0) pzz = pzz + d[i];
+        for (int i = 0; i < 19; i++) {
+          int k = CX[i] * CY[i];
+          if (k > 0) pxy = pxy + d[i]; else if (k < 0) pxy = pxy - d[i];
+        }
+        for (int i = 0; i < 19; i++) {
+          int k = CX[i] * CZ[i];
+          if (k > 0) pxz = pxz + d[i]; else if (k < 0) pxz = pxz - d[i];
+        }
+        for (int i = 0; i < 19; i++) {
+          int k = CY[i] * CZ[i];
+          if (k > 0) pyz = pyz + d[i]; else if (k < 0) pyz = pyz - d[i];
+        }
+        // lbm.py:147-154
+        R qn = std::sqrt(pxx * pxx + pyy * pyy + pzz * pzz +
+                         (R)2.0 * (pxy * pxy + pxz * pxz + pyz * pyz));
+        R tau0 = tau;
+        R sp = (R)x - (R)nx + (R)24.0;
+        if (sp > zero) tau0 = tau0 + sp * (R)0.015;
+        R tau_e = (R)0.5 * (tau0 + std::sqrt(tau0 * tau0 + (R)25.4558441 * smag * smag * qn * ir));
+        R keep = one - one / tau_e;
+        R tr3 = (pxx + pyy + pzz) * ((R)1.0 / (R)3.0);
+        // regularized collision, lbm.py:155-165
+        for (int i = 0; i < 19; i++) {
+          R qp;
+          bool any = false;
+          R acc = zero;
+          if (CX[i]) { acc = pxx; any = true; }
+          if (CY[i]) { acc = any ? acc + pyy : pyy; any = true; }
+          if (CZ[i]) { acc = any ? acc + pzz : pzz; any = true; }
+          if (CX[i] && CY[i]) { R t = (R)(2 * CX[i] * CY[i]) * pxy; acc = any ? acc + t : t; any = true; }
+          if (CX[i] && CZ[i]) { R t = (R)(2 * CX[i] * CZ[i]) * pxz; acc = any ? acc + t : t; any = true; }
+          if (CY[i] && CZ[i]) { R t = (R)(2 * CY[i] * CZ[i]) * pyz; acc = any ? acc + t : t; any = true; }
+          qp = (any ? acc : zero) - tr3;
+          R v = e[i] + (R)(4.5 * w_d(i)) * keep * qp;
+          fout[i * n + idx] = S::store(v, i);
+        }
+        mo[0] = ux; mo[1] = uy; mo[2] = uz; mo[3] = rho;
+        mass += (double)rho;
+      }
+    zmass[z] = mass;
+  }
+  if (mass_out) {
+    double m = 0.0;
+    for (int z = 0; z < nz; z++) m += zmass[z];
+    *mass_out = m;
+  }
+}
+
+// lbm.py:169-197, FP64 accumulation of the per-link terms 2(f_i - w_i).
+template <int FORM>
+void force_t(const void* fv, const uint8_t* solid, int nx, int ny, int nz, double* out) {
+  typedef Form<FORM> S; typedef typename S::real R; typedef typename S::elem E;
+  const E* f = (const E*)fv;
+  const int64_t n = (int64_t)nx * ny * nz;
+  double fx = 0, fy = 0, fz = 0;
+  for (int z = 1; z < nz - 1; z++)
+    for (int y = 1; y < ny - 1; y++)
+      for (int x = 1; x < nx - 1; x++) {
+        int64_t idx = x + (int64_t)nx * (y + (int64_t)ny * z);
+        if (solid[idx] != 0) continue;
+        for (int i = 1; i < 19; i++) {
+          int64_t s = (x + CX[i]) + (int64_t)nx * ((y + CY[i]) + (int64_t)ny * (z + CZ[i]));
+          if (solid[s] != 1) continue;
+          R v = (R)2.0 * (S::load(f[i * n + idx], i) - Lit<R>::w(i));
+          if (CX[i]) fx += CX[i] > 0 ? (double)v : -(double)v;
+          if (CY[i]) fy += CY[i] > 0 ? (double)v : -(double)v;
+          if (CZ[i]) fz += CZ[i] > 0 ? (double)v : -(double)v;
+        }
+      }
+  out[0] = fx; out[1] = fy; out[2] = fz;
+}
+
+// ---- particle kernels (FP32, lbm.py:201-275) --------------------------------
+inline int64_t cidx(int x, int y, int z, int nx, int ny, int nz) {
+  return clampT(x, 0, nx - 1) + (int64_t)nx * (clampT(y, 0, ny - 1) + (int64_t)ny * clampT(z, 0, nz - 1));
+}
+struct V4 { float x, y, z, w; };
+inline V4 ld4(const float* mac, int64_t i) { return V4{mac[4 * i], mac[4 * i + 1], mac[4 * i + 2], mac[4 * i + 3]}; }
+// Warp's lerp(a, b, t) = a*(1-t) + b*t, componentwise.
+inline V4 lerp4(V4 a, V4 b, float t) {
+  float s = 1.0f - t;
+  return V4{a.x * s + b.x * t, a.y * s + b.y * t, a.z * s + b.z * t, a.w * s + b.w * t};
+}
+// lbm.py:206-230
+V4 sample_mac(const float* mac, float px, float py, float pz, int nx, int ny, int nz) {
+  float qx = clampT(px, 0.0f, (float)nx - 1.001f);
+  float qy = clampT(py, 0.0f, (float)ny - 1.001f);
+  float qz = clampT(pz, 0.0f, (float)nz - 1.001f);
+  float bx = std::floor(qx), by = std::floor(qy), bz = std::floor(qz);
+  float tx = qx - bx, ty = qy - by, tz = qz - bz;
+  int ix = (int)bx, iy = (int)by, iz = (int)bz;
+  V4 c000 = ld4(mac, cidx(ix, iy, iz, nx, ny, nz));
+  V4 c100 = ld4(mac, cidx(ix + 1, iy, iz, nx, ny, nz));
+  V4 c010 = ld4(mac, cidx(ix, iy + 1, iz, nx, ny, nz));
+  V4 c110 = ld4(mac, cidx(ix + 1, iy + 1, iz, nx, ny, nz));
+  V4 c001 = ld4(mac, cidx(ix, iy, iz + 1, nx, ny, nz));
+  V4 c101 = ld4(mac, cidx(ix + 1, iy, iz + 1, nx, ny, nz));
+  V4 c011 = ld4(mac, cidx(ix, iy + 1, iz + 1, nx, ny, nz));
+  V4 c111 = ld4(mac, cidx(ix + 1, iy + 1, iz + 1, nx, ny, nz));
+  V4 a = lerp4(lerp4(c000, c100, tx), lerp4(c010, c110, tx), ty);
+  V4 b = lerp4(lerp4(c001, c101, tx), lerp4(c011, c111, tx), ty);
+  return lerp4(a, b, tz);
+}
+
+// Respawn stream: this project's counter-based generator (PCG-RXS-M-XS step on
+// a state seeded by hashing (seed, particle index)).
+inline uint32_t pcg_hash(uint32_t v) {
+  uint32_t s = v * 747796405u + 2891336453u;
+  uint32_t w = ((s >> ((s >> 28u) + 4u)) ^ s) * 277803737u;
+  return (w >> 22u) ^ w;
+}
+inline uint32_t rng_init(uint32_t seed, uint32_t i) { return pcg_hash(seed ^ pcg_hash(i + 0x9e3779b9u)); }
+inline float rng_f(uint32_t& st) {
+  st = pcg_hash(st);
+  return (float)(st >> 8) * (1.0f / 16777216.0f);
+}
+
+}  // namespace
+
+extern "C" {
+
+void oracle_set_threads(int n) { omp_set_num_threads(n > 0 ? n : 1); }
+int oracle_get_threads(void) { return omp_get_max_threads(); }
+
+int oracle_form_elem_bytes(int form) { return form == 0 ? 4 : form == 3 ? 8 : 2; }
+
+void oracle_init(int form, void* f, void* mac, int64_t n, float uin) {
+  switch (form) {
+    case 0: init_t<0>(f, mac, n, uin); break;
+    case 1: init_t<1>(f, mac, n, uin); break;
+    case 2: init_t<2>(f, mac, n, uin); break;
+    case 3: init_t<3>(f, mac, n, uin); break;
+  }
+}
+
+void oracle_step(int form, const void* fin, void* fout, const uint8_t* solid, const void* mac_prev,
+                 void* mac_out, int nx, int ny, int nz, float uin, float tau, float smag,
+                 double* mass) {
+  switch (form) {
+    case 0: step_t<0>(fin, fout, solid, mac_prev, mac_out, nx, ny, nz, uin, tau, smag, mass); break;
+    case 1: step_t<1>(fin, fout, solid, mac_prev, mac_out, nx, ny, nz, uin, tau, smag, mass); break;
+    case 2: step_t<2>(fin, fout, solid, mac_prev, mac_out, nx, ny, nz, uin, tau, smag, mass); break;
+    case 3: step_t<3>(fin, fout, solid, mac_prev, mac_out, nx, ny, nz, uin, tau, smag, mass); break;
+  }
+}
+
+void oracle_force(int form, const void* f, const uint8_t* solid, int nx, int ny, int nz, double* out) {
+  switch (form) {
+    case 0: force_t<0>(f, solid, nx, ny, nz, out); break;
+    case 1: force_t<1>(f, solid, nx, ny, nz, out); break;
+    case 2: force_t<2>(f, solid, nx, ny, nz, out); break;
+    case 3: force_t<3>(f, solid, nx, ny, nz, out); break;
+  }
+}
+
+// Per-cell link classification of lbm.py:81-84, 114-124 and 185-188:
+//   bits 0-1  cell class: 0 interior fluid, 1 solid, 2 face
+//   bit 8+i   (i=1..18) population i is bounced back (solid[idx - c_i] != 0)
+//   bit 27+i  (i=1..18) bounce-back off the moving belt with c_ix != 0 (solid == 2)
+//   bit 45+i  (i=1..18) link i counts toward the force (solid[idx + c_i] == 1)
+// Link bits are set for interior fluid cells only.
+void oracle_link_classes(const uint8_t* solid, int nx, int ny, int nz, uint64_t* out) {
+  for (int z = 0; z < nz; z++)
+    for (int y = 0; y < ny; y++)
+      for (int x = 0; x < nx; x++) {
+        int64_t idx = x + (int64_t)nx * (y + (int64_t)ny * z);
+        uint64_t v;
+        if (solid[idx] != 0) v = 1;
+        else if (x == 0 || x == nx - 1 || y == 0 || y == ny - 1 || z == 0 || z == nz - 1) v = 2;
+        else {
+          v = 0;
+          for (int i = 1; i < 19; i++) {
+            int64_t s = (x - CX[i]) + (int64_t)nx * ((y - CY[i]) + (int64_t)ny * (z - CZ[i]));
+            int t = solid[s];
+            if (t != 0) v |= 1ull << (8 + i);
+            if (t == 2 && CX[i]) v |= 1ull << (27 + i);
+            int64_t d = (x + CX[i]) + (int64_t)nx * ((y + CY[i]) + (int64_t)ny * (z + CZ[i]));
+            if (solid[d] == 1) v |= 1ull << (45 + i);
+          }
+        }
+        out[idx] = v;
+      }
+}
+
+uint32_t oracle_rng_init(uint32_t seed, uint32_t i) { return rng_init(seed, i); }
+float oracle_rng_f(uint32_t* st) { return rng_f(*st); }
+
+// lbm.py:233-263
+void oracle_advect(float* pos, float* spd, const float* mac, const uint8_t* solid, int64_t num_p,
+                   int nx, int ny, int nz, float dt, float uin, float max_age, uint32_t seed,
+                   float seed_x, float seed_cy, float seed_cz, float seed_hy, float seed_hz) {
+  for (int64_t i = 0; i < num_p; i++) {
+    float* p = pos + 4 * i;
+    float qx = p[0], qy = p[1], qz = p[2];
+    V4 m1 = sample_mac(mac, qx, qy, qz, nx, ny, nz);
+    float h = 0.5f * dt;
+    V4 m2 = sample_mac(mac, qx + h * m1.x, qy + h * m1.y, qz + h * m1.z, nx, ny, nz);
+    qx = qx + dt * m2.x; qy = qy + dt * m2.y; qz = qz + dt * m2.z;
+    float age = p[3] + 1.0f;
+    float s = std::sqrt(m2.x * m2.x + m2.y * m2.y + m2.z * m2.z) / uin;
+    bool dead = age > max_age || qx < 0.5f || qy < 0.5f || qz < 0.5f;
+    if (qx > (float)nx - 3.0f || qy > (float)ny - 1.5f || qz > (float)nz - 1.5f || !(s < 100.0f)) dead = true;
+    if (!dead) {
+      int64_t c = cidx((int)std::round(qx), (int)std::round(qy), (int)std::round(qz), nx, ny, nz);
+      if (solid[c] != 0) dead = true;
+    }
+    if (dead) {
+      uint32_t st = rng_init(seed, (uint32_t)i);
+      float r1 = rng_f(st), r2 = rng_f(st), r3 = rng_f(st);
+      qx = seed_x + r1 * 2.0f;
+      qy = seed_cy + (r2 * 2.0f - 1.0f) * seed_hy;
+      qz = seed_cz + (r3 * 2.0f - 1.0f) * seed_hz;
+      age = 0.0f;
+      s = 1.0f;
+    }
+    p[0] = qx; p[1] = qy; p[2] = qz; p[3] = age;
+    spd[i] = s;
+  }
+}
+
+// lbm.py:266-275
+void oracle_pack_particles(const float* pos, const float* spd, uint16_t* q16, uint8_t* q8,
+                           int64_t num_p, int nx, int ny, int nz) {
+  for (int64_t i = 0; i < num_p; i++) {
+    const float* p = pos + 4 * i;
+    q16[4 * i + 0] = (uint16_t)(int)(clampT(p[0] / (float)nx, 0.0f, 1.0f) * 65535.0f + 0.5f);
+    q16[4 * i + 1] = (uint16_t)(int)(clampT(p[1] / (float)ny, 0.0f, 1.0f) * 65535.0f + 0.5f);
+    q16[4 * i + 2] = (uint16_t)(int)(clampT(p[2] / (float)nz, 0.0f, 1.0f) * 65535.0f + 0.5f);
+    q16[4 * i + 3] = (uint16_t)(int)clampT(p[3], 0.0f, 65535.0f);
+    q8[i] = (uint8_t)(int)(clampT(spd[i] / 2.0f, 0.0f, 1.0f) * 255.0f + 0.5f);
+  }
+}
+
+// lbm.py:278-318
+void oracle_slice(const float* mac, const uint8_t* solid, uint8_t* out, int nx, int ny, int nz,
+                  int axis, int index, int field, float uin, float lref, float vmin, float vmax) {
+  int w = axis == 2 ? nx : (axis == 1 ? nx : nz);
+  int h = axis == 2 ? ny : (axis == 1 ? nz : ny);
+  for (int64_t k = 0; k < (int64_t)w * h; k++) {
+    int a = (int)(k % w), b = (int)(k / w);
+    int cx = index, cy = b, cz = a;
+    if (axis == 2) { cx = a; cy = b; cz = index; }
+    else if (axis == 1) { cx = a; cy = index; cz = b; }
+    int64_t idx = cx + (int64_t)nx * (cy + (int64_t)ny * cz);
+    V4 m = ld4(mac, idx);
+    float v = m.x / uin;
+    if (field == 0) {
+      v = std::sqrt(m.x * m.x + m.y * m.y + m.z * m.z) / uin;
+    } else if (field == 1) {
+      v = (m.w - 1.0f) / (1.5f * uin * uin);
+    } else if (field == 2) {
+      V4 xp = ld4(mac, cidx(cx + 1, cy, cz, nx, ny, nz)), xm = ld4(mac, cidx(cx - 1, cy, cz, nx, ny, nz));
+      V4 yp = ld4(mac, cidx(cx, cy + 1, cz, nx, ny, nz)), ym = ld4(mac, cidx(cx, cy - 1, cz, nx, ny, nz));
+      V4 zp = ld4(mac, cidx(cx, cy, cz + 1, nx, ny, nz)), zm = ld4(mac, cidx(cx, cy, cz - 1, nx, ny, nz));
+      float dxy = (xp.y - xm.y) * 0.5f, dxz = (xp.z - xm.z) * 0.5f;
+      float dyx = (yp.x - ym.x) * 0.5f, dyz = (yp.z - ym.z) * 0.5f;
+      float dzx = (zp.x - zm.x) * 0.5f, dzy = (zp.y - zm.y) * 0.5f;
+      float wx = dyz - dzy, wy = dzx - dxz, wz = dxy - dyx;
+      v = std::sqrt(wx * wx + wy * wy + wz * wz) * lref / uin;
+    }
+    float t = clampT((v - vmin) / (vmax - vmin), 0.0f, 1.0f);
+    out[2 * k] = (uint8_t)(int)(t * 255.0f + 0.5f);
+    out[2 * k + 1] = solid[idx] != 0 ? 0 : 255;
+  }
+}
+
+// lbm.py:321-350
+static void rho_fluid(const float* mac, const uint8_t* solid, float px, float py, float pz, int nx,
+                      int ny, int nz, float* a0, float* a1) {
+  float qx = clampT(px, 0.0f, (float)nx - 1.001f);
+  float qy = clampT(py, 0.0f, (float)ny - 1.001f);
+  float qz = clampT(pz, 0.0f, (float)nz - 1.001f);
+  int ix = (int)std::floor(qx), iy = (int)std::floor(qy), iz = (int)std::floor(qz);
+  float tx = qx - (float)ix, ty = qy - (float)iy, tz = qz - (float)iz;
+  float s0 = 0.0f, s1 = 0.0f;
+  for (int c = 0; c < 8; c++) {
+    int ox = c & 1, oy = (c >> 1) & 1, oz = (c >> 2) & 1;
+    int64_t ci = cidx(ix + ox, iy + oy, iz + oz, nx, ny, nz);
+    float wx = ox ? tx : 1.0f - tx;
+    float wy = oy ? ty : 1.0f - ty;
+    float wz = oz ? tz : 1.0f - tz;
+    float wt = wx * wy * wz;
+    if (solid[ci] == 0) { s0 = s0 + wt * mac[4 * ci + 3]; s1 = s1 + wt; }
+  }
+  *a0 = s0; *a1 = s1;
+}
+
+// lbm.py:353-368
+void oracle_cp(const float* mac, const uint8_t* solid, const float* verts, const float* norms,
+               int8_t* out, int64_t nv, int nx, int ny, int nz, float uin, float cp_range) {
+  for (int64_t i = 0; i < nv; i++) {
+    const float* p = verts + 3 * i;
+    const float* r = norms + 3 * i;
+    float a0, a1, b0, b1;
+    rho_fluid(mac, solid, p[0] + r[0] * 1.25f, p[1] + r[1] * 1.25f, p[2] + r[2] * 1.25f, nx, ny, nz, &a0, &a1);
+    rho_fluid(mac, solid, p[0] - r[0] * 1.25f, p[1] - r[1] * 1.25f, p[2] - r[2] * 1.25f, nx, ny, nz, &b0, &b1);
+    float s0 = b0, s1 = b1;
+    if (a1 >= b1) { s0 = a0; s1 = a1; }
+    float cp = 0.0f;
+    if (s1 > 0.05f) cp = (s0 / s1 - 1.0f) / (1.5f * uin * uin);
+    out[i] = (int8_t)(int)std::round(clampT(cp / cp_range, -1.0f, 1.0f) * 127.0f);
+  }
+}
+
+// Conversions exposed for tests of the storage forms.
+uint16_t oracle_f2h(float f) { return f2h(f); }
+float oracle_h2f(uint16_t h) { return h2f(h); }
+uint16_t oracle_f2bf(float f) { return f2bf(f); }
+float oracle_bf2f(uint16_t b) { return bf2f(b); }
+
+}  // extern "C"

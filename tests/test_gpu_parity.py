@@ -1,0 +1,261 @@
+"""Parity of the HIP kernels (through the C ABI) with the CPU oracle.
+
+Bar: bit-exact for integer outputs (link classes, particle cell decisions,
+quantized bytes) and, because both sides round every IEEE operation once in
+the same order, bit-exact populations and macroscopic fields for every storage
+form. Where FP64 sums are reduced in a different order (forces, mass) the
+tolerance is rtol 1e-12. The config-1 field test also states a physical
+tolerance (|du| <= 1e-6, |drho| <= 1e-6 over 100 steps) in case a future
+fast-math variant stops being bit-exact.
+"""
+import numpy as np
+import pytest
+import torch
+
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+
+def _tunnel(dims, form, mask, uin):
+    from windtunnel.tunnel import Tunnel
+    t = Tunnel(dims, form)
+    t.init(uin)
+    t.set_solid(mask)
+    return t
+
+
+def _oracle(dims, form, mask, uin):
+    o = O.OracleTunnel(dims, form)
+    o.init(uin)
+    o.set_solid(mask)
+    return o
+
+
+def _car_mask(dims, seed=1, ground=True):
+    from windtunnel import geometry
+    nx, ny, nz = dims
+    rng = np.random.default_rng(seed)
+    k = 5
+    ells = np.column_stack([rng.uniform(0.25 * nx, 0.5 * nx, k), rng.uniform(2, 0.4 * ny, k),
+                            rng.uniform(0.35 * nz, 0.65 * nz, k), rng.uniform(1.5, 0.12 * nx, k),
+                            rng.uniform(1.5, 0.25 * ny, k), rng.uniform(1.5, 0.25 * nz, k)])
+    return geometry.ellipsoids(ells, dims, ground=ground)[0]
+
+
+def _sphere_mask(dims, c, d):
+    from windtunnel import geometry
+    return geometry.sphere(dims, c, d)[0]
+
+
+def _raw_equal(a, b):
+    return np.array_equal(np.ascontiguousarray(a).view(np.uint8), np.ascontiguousarray(b).view(np.uint8))
+
+
+# ---- link classification ---------------------------------------------------------------------------
+@pytest.mark.parametrize('kind', ['sphere', 'car_belt', 'random_codes'])
+def test_link_classes_bit_exact(gpu, kind):
+    dims = (40, 22, 18)
+    if kind == 'sphere':
+        mask = _sphere_mask(dims, (14, 11, 9), 9)
+    elif kind == 'car_belt':
+        mask = _car_mask(dims)
+    else:
+        rng = np.random.default_rng(4)
+        mask = rng.choice(np.array([0, 0, 0, 0, 1, 2, 3, 200], np.uint8), size=int(np.prod(dims)))
+    t = _tunnel(dims, 'fp32', mask, 0.05)
+    got = t.link_classes().cpu().numpy().view(np.uint64)
+    want = O.link_classes(mask, dims)
+    assert np.array_equal(got, want)
+
+
+# ---- the fused step -------------------------------------------------------------------------------
+FORMS = [('fp32', O.FP32), ('fp16', O.FP16), ('bf16', O.BF16), ('fp64', O.FP64)]
+
+
+@pytest.mark.parametrize('name,form', FORMS)
+def test_step_bit_exact_with_forces_and_mass(gpu, name, form):
+    from windtunnel._lib import WT_FLAG_FORCES, WT_FLAG_MASS
+    dims = (50, 20, 18)
+    mask = _car_mask(dims, seed=2)
+    uin, tau, smag = 0.06, 0.515, 0.17
+    t = _tunnel(dims, name, mask, uin)
+    o = _oracle(dims, form, mask, uin)
+    for chunk in (1, 7, 12):
+        t.step(chunk, uin, tau, smag, flags=WT_FLAG_FORCES | WT_FLAG_MASS)
+        o.step(chunk, uin, tau, smag)
+        torch.cuda.synchronize()
+        assert t.cur == o.cur
+        assert _raw_equal(t.raw_populations().cpu().numpy(), o.raw()), (name, chunk)
+        mac = t.mac.cpu().numpy()
+        assert _raw_equal(mac, o.mac.astype(np.float32)), (name, chunk)
+        sums = t.sums.cpu().numpy()
+        np.testing.assert_allclose(sums[:3], o.forces(), rtol=1e-12, atol=1e-15)
+        np.testing.assert_allclose(sums[3], o.mass, rtol=1e-12)
+
+
+def test_config1_sphere_100_steps(gpu):
+    """Config 1: 128x64x64, sphere D=16 at (40,32,32), uin 0.05, tau 0.52, Cs 0.17."""
+    from windtunnel._lib import WT_FLAG_FORCES
+    dims = (128, 64, 64)
+    mask = _sphere_mask(dims, (40, 32, 32), 16)
+    t = _tunnel(dims, 'fp32', mask, 0.05)
+    o = _oracle(dims, O.FP32, mask, 0.05)
+    fx_gpu, fx_cpu = [], []
+    for k in range(10):
+        t.step(10, 0.05, 0.52, 0.17, flags=WT_FLAG_FORCES)
+        o.step(10, 0.05, 0.52, 0.17)
+        fx_gpu.append(t.sums.cpu().numpy()[:3].copy())
+        fx_cpu.append(o.forces())
+    mac_g = t.mac.cpu().numpy().astype(np.float64)
+    mac_c = o.mac.astype(np.float64)
+    du = np.abs(mac_g[:, :3] - mac_c[:, :3]).max()
+    dr = np.abs(mac_g[:, 3] - mac_c[:, 3]).max()
+    print(f'config-1 100 steps: max|du|={du:.3e} max|drho|={dr:.3e}')
+    assert du <= 1e-6 and dr <= 1e-6
+    np.testing.assert_allclose(np.array(fx_gpu), np.array(fx_cpu), rtol=1e-12, atol=1e-14)
+    assert _raw_equal(t.raw_populations().cpu().numpy(), o.raw())
+    assert _raw_equal(t.mac.cpu().numpy(), o.mac)
+
+
+def test_fused_force_equals_standalone(gpu):
+    from windtunnel._lib import WT_FLAG_FORCES
+    dims = (64, 24, 24)
+    mask = _car_mask(dims, seed=5)
+    t = _tunnel(dims, 'fp32', mask, 0.05)
+    t.step(9, 0.05, 0.52, 0.16, flags=WT_FLAG_FORCES)
+    fused = t.sums.cpu().numpy().copy()
+    t.forces()
+    alone = t.sums.cpu().numpy().copy()
+    assert _raw_equal(fused[:3], alone[:3])
+
+
+@pytest.mark.parametrize('cpt,remap', [(1, 0), (2, 1), (4, 1), (4, 0)])
+def test_launch_shape_does_not_change_results(gpu, cpt, remap):
+    from windtunnel import tunnel as T
+    from windtunnel._lib import WT_FLAG_FORCES, WT_FLAG_MASS
+    dims = (70, 26, 22)    # n not a multiple of 256 or 1024
+    mask = _car_mask(dims, seed=6)
+    old = T.get_launch_shape('fp32')
+    res = []
+    try:
+        for shape in ((1, 1), (cpt, remap)):
+            T.set_launch_shape('fp32', *shape)
+            t = _tunnel(dims, 'fp32', mask, 0.05)
+            t.step(11, 0.05, 0.52, 0.17, flags=WT_FLAG_FORCES | WT_FLAG_MASS)
+            res.append((t.raw_populations().cpu().numpy(), t.mac.cpu().numpy(), t.sums.cpu().numpy()))
+    finally:
+        T.set_launch_shape('fp32', *old)
+    for a, b in zip(res[0], res[1]):
+        assert _raw_equal(a, b)
+
+
+def test_repeatable(gpu):
+    from windtunnel._lib import WT_FLAG_FORCES
+    dims = (48, 20, 20)
+    mask = _car_mask(dims, seed=7)
+    out = []
+    for _ in range(2):
+        t = _tunnel(dims, 'fp16', mask, 0.05)
+        t.step(15, 0.05, 0.52, 0.17, flags=WT_FLAG_FORCES)
+        out.append((t.raw_populations().cpu().numpy(), t.sums.cpu().numpy()))
+    assert _raw_equal(out[0][0], out[1][0]) and _raw_equal(out[0][1], out[1][1])
+
+
+# ---- particles, slices, surface pressure -------------------------------------------------------
+def _flow_state(dims=(60, 24, 22), steps=25):
+    mask = _car_mask(dims, seed=8)
+    t = _tunnel(dims, 'fp32', mask, 0.05)
+    t.step(steps, 0.05, 0.52, 0.17)
+    return t, mask
+
+
+def test_advect_bit_exact(gpu):
+    t, mask = _flow_state()
+    dims = (t.nx, t.ny, t.nz)
+    rng = np.random.default_rng(3)
+    N = 20000
+    pos = np.zeros((N, 4), np.float32)
+    pos[:, 0] = rng.uniform(-1, t.nx + 1, N)
+    pos[:, 1] = rng.uniform(-1, t.ny + 1, N)
+    pos[:, 2] = rng.uniform(-1, t.nz + 1, N)
+    pos[:, 3] = rng.uniform(0, 120, N)
+    spd = np.ones(N, np.float32)
+    box = (2.0, 10.0, 11.0, 6.0, 5.0)
+    gp = torch.from_numpy(pos.copy()).cuda()
+    gs = torch.from_numpy(spd.copy()).cuda()
+    mac = t.mac.cpu().numpy()
+    cp_, cs_ = pos.copy(), spd.copy()
+    for k in range(4):
+        t.advect(gp, gs, 3.0, 0.05, 100.0, 40 + k, box)
+        cp_, cs_ = O.advect(cp_, cs_, mac, mask, dims, 3.0, 0.05, 100.0, 40 + k, box)
+    assert _raw_equal(gp.cpu().numpy(), cp_) and _raw_equal(gs.cpu().numpy(), cs_)
+
+
+def test_pack_bit_exact(gpu):
+    t, _ = _flow_state(steps=2)
+    rng = np.random.default_rng(5)
+    N = 5000
+    pos = np.column_stack([rng.uniform(-5, 70, N), rng.uniform(-5, 30, N), rng.uniform(-5, 30, N),
+                           rng.uniform(-10, 70000, N)]).astype(np.float32)
+    spd = rng.uniform(-1, 3, N).astype(np.float32)
+    q16 = torch.zeros(4 * N, dtype=torch.int16, device='cuda')
+    q8 = torch.zeros(N, dtype=torch.uint8, device='cuda')
+    t.pack_particles(torch.from_numpy(pos).cuda(), torch.from_numpy(spd).cuda(), q16, q8)
+    e16, e8 = O.pack_particles(pos, spd, (t.nx, t.ny, t.nz))
+    assert np.array_equal(q16.cpu().numpy().view(np.uint16), e16)
+    assert np.array_equal(q8.cpu().numpy(), e8)
+
+
+@pytest.mark.parametrize('axis,field', [(2, 0), (2, 1), (2, 2), (2, 3), (1, 2), (0, 0), (0, 2)])
+def test_slice_bit_exact(gpu, axis, field):
+    t, mask = _flow_state()
+    dims = (t.nx, t.ny, t.nz)
+    ranges = {0: (0, 1.6), 1: (-1.2, 1.2), 2: (0, 12), 3: (-0.4, 1.4)}
+    vmin, vmax = ranges[field]
+    index = dims[axis] // 2 - 1
+    w, h = O.slice_dims(dims, axis)
+    out = torch.zeros(2 * w * h, dtype=torch.uint8, device='cuda')
+    t.slice(out, axis, index, field, 0.05, 30.0, vmin, vmax)
+    want = O.slice_(t.mac.cpu().numpy(), mask, dims, axis, index, field, 0.05, 30.0, vmin, vmax)
+    assert np.array_equal(out.cpu().numpy(), want)
+
+
+def test_cp_bit_exact(gpu):
+    t, mask = _flow_state()
+    dims = (t.nx, t.ny, t.nz)
+    rng = np.random.default_rng(2)
+    nv = 3000
+    verts = np.column_stack([rng.uniform(0, t.nx, nv), rng.uniform(0, t.ny, nv),
+                             rng.uniform(0, t.nz, nv)]).astype(np.float32)
+    nrm = rng.standard_normal((nv, 3))
+    nrm = (nrm / np.linalg.norm(nrm, axis=1, keepdims=True)).astype(np.float32)
+    out = torch.zeros(nv, dtype=torch.int8, device='cuda')
+    t.cp(torch.from_numpy(verts).cuda(), torch.from_numpy(nrm).cuda(), out, 0.05, 1.2)
+    want = O.cp(t.mac.cpu().numpy(), mask, verts, nrm, dims, 0.05, 1.2)
+    assert np.array_equal(out.cpu().numpy(), want)
+
+
+# ---- edge cases ---------------------------------------------------------------------------------------
+def test_smallest_grid_and_all_solid(gpu):
+    dims = (3, 3, 3)
+    for mask in (np.zeros(27, np.uint8), np.ones(27, np.uint8)):
+        t = _tunnel(dims, 'fp32', mask, 0.05)
+        o = _oracle(dims, O.FP32, mask, 0.05)
+        t.step(3, 0.05, 0.52, 0.17)
+        o.step(3, 0.05, 0.52, 0.17)
+        assert _raw_equal(t.raw_populations().cpu().numpy(), o.raw())
+        assert _raw_equal(t.mac.cpu().numpy(), o.mac)
+
+
+def test_rejects_bad_arguments(gpu):
+    from windtunnel.tunnel import Tunnel
+    with pytest.raises(RuntimeError, match='at least 3'):
+        Tunnel((2, 8, 8))
+    t = Tunnel((8, 8, 8))
+    t.init(0.05)
+    out = torch.zeros(128, dtype=torch.uint8, device='cuda')
+    with pytest.raises(RuntimeError, match='slice index'):
+        t.slice(out, 2, 8, 0, 0.05, 40.0, 0.0, 1.6)
+    with pytest.raises(ValueError):
+        t.set_solid(np.zeros(10, np.uint8))

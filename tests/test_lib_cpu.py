@@ -1,0 +1,157 @@
+"""Host-side checks of libwt.so that need no GPU: the C ABI loads and exports
+every entry point include/wt.h declares, the host geometry matches the
+reference voxelizer bit for bit, and the size/admission arithmetic holds."""
+import ctypes
+import json
+import os
+import re
+import subprocess
+import zlib
+
+import numpy as np
+import pytest
+
+REPO = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+HEADER = os.path.join(REPO, 'include', 'wt.h')
+GOLDEN = os.path.join(REPO, 'tests', 'golden')
+
+
+def declared_functions():
+    text = open(HEADER).read()
+    text = re.sub(r'/\*.*?\*/', '', text, flags=re.S)
+    return sorted(set(re.findall(r'^\s*(?:const\s+)?[a-z_0-9]+\s*\*?\s*(wt_[a-z_0-9]+)\s*\(', text, flags=re.M)))
+
+
+def test_library_exports_every_declared_symbol():
+    from windtunnel import _lib
+    lib = _lib.load()
+    names = declared_functions()
+    assert len(names) >= 20, names
+    out = subprocess.run(['nm', '-D', '--defined-only', _lib.LIB_PATH], capture_output=True, text=True,
+                         check=True).stdout
+    exported = set(l.split()[-1] for l in out.splitlines() if ' T ' in l)
+    missing = [n for n in names if n not in exported]
+    assert not missing, missing
+    for n in names:
+        assert hasattr(lib, n)
+    # every declared function has a ctypes signature in the binding, and vice versa
+    assert sorted(_lib.SIGNATURES) == names
+
+
+def test_error_reporting_without_gpu_calls():
+    from windtunnel import _lib
+    lib = _lib.load()
+    s = (ctypes.c_int64 * 6)()
+    assert lib.wt_buffer_sizes(2, 8, 8, 0, s) < 0
+    assert 'at least 3' in _lib.last_error()
+    assert lib.wt_buffer_sizes(8, 8, 8, 9, s) < 0
+    assert 'storage form' in _lib.last_error()
+    assert lib.wt_set_launch_shape(0, 3, 0) < 0
+    assert lib.wt_version() == 1
+
+
+@pytest.mark.parametrize('form,elem', [(0, 4), (1, 2), (2, 2), (3, 8)])
+def test_buffer_sizes_and_footprint(form, elem):
+    from windtunnel import _lib
+    lib = _lib.load()
+    nx, ny, nz = 512, 256, 256
+    n = nx * ny * nz
+    s = (ctypes.c_int64 * 6)()
+    assert lib.wt_buffer_sizes(nx, ny, nz, form, s) == 0
+    assert s[0] == 19 * n * elem and s[1] == n and s[2] == 16 * n
+    shell = 2 * (ny * nz + nx * nz + nx * ny) * 4 * (8 if form == 3 else 4)
+    assert s[3] == shell
+    total = lib.wt_state_bytes(nx, ny, nz, form)
+    assert total == 2 * s[0] + s[1] + s[2] + 2 * s[3] + s[4] + s[5]
+    # per cell: 2 x 19 populations + 16 B mac + 1 B class, plus O(n^(2/3)) extras
+    per_cell = 2 * 19 * elem + 17
+    assert per_cell * n <= total <= per_cell * n * 1.02
+    if form == 0:
+        assert per_cell == 169      # lbm.py:30 BYTES_PER_CELL
+
+
+def _golden_cases():
+    meta = json.load(open(os.path.join(GOLDEN, 'voxel_golden.json')))
+    blob = open(os.path.join(GOLDEN, 'voxel_golden.bin'), 'rb').read()
+    for c in meta['cases']:
+        tris = np.frombuffer(blob[c['tris_offset']:c['tris_offset'] + c['tris_bytes']], np.float32)
+        mask = np.frombuffer(zlib.decompress(blob[c['mask_offset']:c['mask_offset'] + c['mask_zbytes']]), np.uint8)
+        yield c, tris, mask
+
+
+@pytest.mark.parametrize('case', [c['name'] for c, _, _ in _golden_cases()])
+def test_voxelize_matches_reference_bit_for_bit(case):
+    """Masks and counts of web/js/voxelize.js, produced by the reference itself under
+    Node (tests/golden/make_voxel_golden.js)."""
+    from windtunnel import geometry
+    for c, tris, mask in _golden_cases():
+        if c['name'] != case:
+            continue
+        got, counts = geometry.voxelize(tris, c['dims'], c['ground'])
+        assert got.shape == mask.shape
+        bad = np.flatnonzero(got != mask)
+        assert bad.size == 0, (case, bad[:10], got[bad[:10]], mask[bad[:10]])
+        assert counts == {'count': c['count'], 'frontal': c['frontal'], 'planform': c['planform']}
+
+
+def test_sphere_counts_closed_form():
+    """Sphere D=16 at (40,32,32), cell centres with the <= r^2 rule: 2,109 cells and a
+    frontal area of 197 = the Gauss circle count N(8) (lattice points with y^2+z^2 <= 64)."""
+    from windtunnel import geometry
+    dims = (128, 64, 64)
+    mask, counts = geometry.sphere(dims, (40, 32, 32), 16)
+    assert counts['count'] == 2109 and mask.sum() == 2109
+    assert counts['frontal'] == sum(1 for y in range(-8, 9) for z in range(-8, 9) if y * y + z * z <= 64) == 197
+    assert counts['planform'] == 197
+    z, y, x = np.meshgrid(np.arange(64), np.arange(64), np.arange(128), indexing='ij')
+    brute = ((x - 40) ** 2 + (y - 32) ** 2 + (z - 32) ** 2 <= 64).astype(np.uint8).reshape(-1)
+    assert np.array_equal(mask, brute)
+
+
+def test_ellipsoid_union_brute_force_and_closure():
+    from windtunnel import geometry
+    dims = (60, 30, 34)
+    nx, ny, nz = dims
+    rng = np.random.default_rng(9)
+    ells = np.column_stack([rng.uniform(15, 45, 6), rng.uniform(5, 25, 6), rng.uniform(8, 26, 6),
+                            rng.uniform(2, 9, (6, 3))])
+    mask, counts = geometry.ellipsoids(ells, dims, ground=True)
+    z, y, x = np.meshgrid(np.arange(nz), np.arange(ny), np.arange(nx), indexing='ij')
+    inside = np.zeros((nz, ny, nx), bool)
+    for cx, cy, cz, ax, ay, az in ells:
+        dx, dy, dz = (x - cx) / ax, (y - cy) / ay, (z - cz) / az
+        inside |= dx * dx + dy * dy + dz * dz <= 1.0
+    inside[[0, -1], :, :] = False
+    inside[:, [0, -1], :] = False
+    inside[:, :, [0, -1]] = False
+    m3 = mask.reshape(nz, ny, nx)
+    body = m3 == 1
+    # every union cell is solid; extra solid cells are only enclosed pockets
+    assert np.all(body[inside])
+    from scipy import ndimage
+    fluid = ~inside
+    lab, _ = ndimage.label(fluid)
+    border = set(np.unique(np.concatenate([lab[0].ravel(), lab[-1].ravel(), lab[:, 0].ravel(), lab[:, -1].ravel(),
+                                            lab[:, :, 0].ravel(), lab[:, :, -1].ravel()])))
+    enclosed = fluid & ~np.isin(lab, list(border))
+    expect = inside | enclosed
+    expect[:, 0, :] = False
+    assert np.array_equal(body, expect)
+    assert np.all(m3[:, 0, :] == 2)
+    assert counts['count'] == int((inside | enclosed).sum())
+
+
+def test_closure_fills_enclosed_pocket():
+    """A hollow ellipsoid shell (union of 6 slabs) encloses fluid that the flood fill closes."""
+    from windtunnel import geometry
+    dims = (30, 30, 30)
+    ells = []
+    for ax in range(3):
+        for s in (-1, 1):
+            c = [15.0, 15.0, 15.0]
+            c[ax] += s * 6
+            a = [8.0, 8.0, 8.0]
+            a[ax] = 1.2
+            ells.append(c + a)
+    mask, counts = geometry.ellipsoids(ells, dims)
+    assert mask[15 + 30 * (15 + 30 * 15)] == 1     # centre was fluid, enclosed -> solid

@@ -1,0 +1,317 @@
+// Particle tracer, particle quantizer, slice extractor and surface-pressure
+// kernels, plus the library's error state and device query.
+//
+// Each kernel restates one reference kernel operation by operation; with the
+// library compiled -ffp-contract=off, the integer outputs (cell indices, kill
+// decisions, quantized bytes) are identical to a CPU evaluation of the same
+// IEEE operations on the same inputs.
+#include <hip/hip_runtime.h>
+
+#include <cstring>
+
+#include "wt_device.h"
+#include "wt_host.h"
+
+namespace wt {
+
+char* error_buffer() {
+  static thread_local char buf[512] = {0};
+  return buf;
+}
+
+constexpr int ABLOCK = 256;
+
+__device__ __forceinline__ float clampf(float v, float a, float b) {
+  return (v < a) ? a : ((v > b) ? b : v);
+}
+__device__ __forceinline__ int clampi(int v, int a, int b) { return (v < a) ? a : ((v > b) ? b : v); }
+
+// lbm.py:201-203
+__device__ __forceinline__ uint32_t cidx(int x, int y, int z, int nx, int ny, int nz) {
+  return (uint32_t)clampi(x, 0, nx - 1) + (uint32_t)nx * ((uint32_t)clampi(y, 0, ny - 1) +
+                                                          (uint32_t)ny * (uint32_t)clampi(z, 0, nz - 1));
+}
+
+// Warp's lerp(a, b, t) = a*(1-t) + b*t, componentwise.
+__device__ __forceinline__ float4 lerp4(float4 a, float4 b, float t) {
+  const float s = 1.0f - t;
+  return make_float4(a.x * s + b.x * t, a.y * s + b.y * t, a.z * s + b.z * t, a.w * s + b.w * t);
+}
+
+// Trilinear sample of the macroscopic field, lbm.py:206-230.
+__device__ float4 sample_mac(const float4* __restrict__ mac, float px, float py, float pz, int nx, int ny,
+                             int nz) {
+  const float qx = clampf(px, 0.0f, (float)nx - 1.001f);
+  const float qy = clampf(py, 0.0f, (float)ny - 1.001f);
+  const float qz = clampf(pz, 0.0f, (float)nz - 1.001f);
+  const float bx = floorf(qx), by = floorf(qy), bz = floorf(qz);
+  const float tx = qx - bx, ty = qy - by, tz = qz - bz;
+  const int ix = (int)bx, iy = (int)by, iz = (int)bz;
+  const float4 c000 = mac[cidx(ix, iy, iz, nx, ny, nz)];
+  const float4 c100 = mac[cidx(ix + 1, iy, iz, nx, ny, nz)];
+  const float4 c010 = mac[cidx(ix, iy + 1, iz, nx, ny, nz)];
+  const float4 c110 = mac[cidx(ix + 1, iy + 1, iz, nx, ny, nz)];
+  const float4 c001 = mac[cidx(ix, iy, iz + 1, nx, ny, nz)];
+  const float4 c101 = mac[cidx(ix + 1, iy, iz + 1, nx, ny, nz)];
+  const float4 c011 = mac[cidx(ix, iy + 1, iz + 1, nx, ny, nz)];
+  const float4 c111 = mac[cidx(ix + 1, iy + 1, iz + 1, nx, ny, nz)];
+  const float4 a = lerp4(lerp4(c000, c100, tx), lerp4(c010, c110, tx), ty);
+  const float4 b = lerp4(lerp4(c001, c101, tx), lerp4(c011, c111, tx), ty);
+  return lerp4(a, b, tz);
+}
+
+// Respawn stream: counter-based, PCG-RXS-M-XS output step applied to a state
+// derived from (seed, particle index). Same definition as the CPU oracle.
+__device__ __forceinline__ uint32_t pcg_hash(uint32_t v) {
+  const uint32_t s = v * 747796405u + 2891336453u;
+  const uint32_t w = ((s >> ((s >> 28u) + 4u)) ^ s) * 277803737u;
+  return (w >> 22u) ^ w;
+}
+__device__ __forceinline__ uint32_t rng_init(uint32_t seed, uint32_t i) {
+  return pcg_hash(seed ^ pcg_hash(i + 0x9e3779b9u));
+}
+__device__ __forceinline__ float rng_f(uint32_t& st) {
+  st = pcg_hash(st);
+  return (float)(st >> 8) * (1.0f / 16777216.0f);
+}
+
+struct AdvectArgs {
+  const float4* mac;
+  const uint8_t* cls;
+  float4* pos;
+  float* spd;
+  uint32_t num_p;
+  int nx, ny, nz;
+  float dt, uin, max_age;
+  uint32_t seed;
+  wt_seed_box box;
+};
+
+// lbm.py:233-263
+__global__ __launch_bounds__(ABLOCK) void k_advect(AdvectArgs a) {
+  const uint32_t i = blockIdx.x * ABLOCK + threadIdx.x;
+  if (i >= a.num_p) return;
+  const float4 p = a.pos[i];
+  float qx = p.x, qy = p.y, qz = p.z;
+  const float4 m1 = sample_mac(a.mac, qx, qy, qz, a.nx, a.ny, a.nz);
+  const float h = 0.5f * a.dt;
+  const float4 m2 = sample_mac(a.mac, qx + h * m1.x, qy + h * m1.y, qz + h * m1.z, a.nx, a.ny, a.nz);
+  qx = qx + a.dt * m2.x;
+  qy = qy + a.dt * m2.y;
+  qz = qz + a.dt * m2.z;
+  float age = p.w + 1.0f;
+  float s = sqrtf(m2.x * m2.x + m2.y * m2.y + m2.z * m2.z) / a.uin;
+  bool dead = age > a.max_age || qx < 0.5f || qy < 0.5f || qz < 0.5f;
+  if (qx > (float)a.nx - 3.0f || qy > (float)a.ny - 1.5f || qz > (float)a.nz - 1.5f || !(s < 100.0f))
+    dead = true;
+  if (!dead) {
+    const uint32_t c = cidx((int)roundf(qx), (int)roundf(qy), (int)roundf(qz), a.nx, a.ny, a.nz);
+    if ((a.cls[c] & CLS_CODE) != 0) dead = true;
+  }
+  if (dead) {
+    uint32_t st = rng_init(a.seed, i);
+    const float r1 = rng_f(st), r2 = rng_f(st), r3 = rng_f(st);
+    qx = a.box.x + r1 * 2.0f;
+    qy = a.box.cy + (r2 * 2.0f - 1.0f) * a.box.hy;
+    qz = a.box.cz + (r3 * 2.0f - 1.0f) * a.box.hz;
+    age = 0.0f;
+    s = 1.0f;
+  }
+  a.pos[i] = make_float4(qx, qy, qz, age);
+  a.spd[i] = s;
+}
+
+// lbm.py:266-275
+__global__ __launch_bounds__(ABLOCK) void k_pack(const float4* __restrict__ pos, const float* __restrict__ spd,
+                                                 ushort4* q16, uint8_t* q8, uint32_t num_p, int nx, int ny,
+                                                 int nz) {
+  const uint32_t i = blockIdx.x * ABLOCK + threadIdx.x;
+  if (i >= num_p) return;
+  const float4 p = pos[i];
+  ushort4 q;
+  q.x = (uint16_t)(int)(clampf(p.x / (float)nx, 0.0f, 1.0f) * 65535.0f + 0.5f);
+  q.y = (uint16_t)(int)(clampf(p.y / (float)ny, 0.0f, 1.0f) * 65535.0f + 0.5f);
+  q.z = (uint16_t)(int)(clampf(p.z / (float)nz, 0.0f, 1.0f) * 65535.0f + 0.5f);
+  q.w = (uint16_t)(int)clampf(p.w, 0.0f, 65535.0f);
+  q16[i] = q;
+  q8[i] = (uint8_t)(int)(clampf(spd[i] / 2.0f, 0.0f, 1.0f) * 255.0f + 0.5f);
+}
+
+struct SliceArgs {
+  const float4* mac;
+  const uint8_t* cls;
+  uint8_t* out;
+  int nx, ny, nz, axis, index, field, w;
+  uint32_t npix;
+  float uin, lref, vmin, vmax;
+};
+
+// lbm.py:278-318
+__global__ __launch_bounds__(ABLOCK) void k_slice(SliceArgs a) {
+  const uint32_t k = blockIdx.x * ABLOCK + threadIdx.x;
+  if (k >= a.npix) return;
+  const int pa = (int)(k % (uint32_t)a.w), pb = (int)(k / (uint32_t)a.w);
+  int cx = a.index, cy = pb, cz = pa;
+  if (a.axis == 2) { cx = pa; cy = pb; cz = a.index; }
+  else if (a.axis == 1) { cx = pa; cy = a.index; cz = pb; }
+  const uint32_t idx = (uint32_t)cx + (uint32_t)a.nx * ((uint32_t)cy + (uint32_t)a.ny * (uint32_t)cz);
+  const float4 m = a.mac[idx];
+  float v = m.x / a.uin;
+  if (a.field == 0) {
+    v = sqrtf(m.x * m.x + m.y * m.y + m.z * m.z) / a.uin;
+  } else if (a.field == 1) {
+    v = (m.w - 1.0f) / (1.5f * a.uin * a.uin);
+  } else if (a.field == 2) {
+    const int nx = a.nx, ny = a.ny, nz = a.nz;
+    const float4 xp = a.mac[cidx(cx + 1, cy, cz, nx, ny, nz)], xm = a.mac[cidx(cx - 1, cy, cz, nx, ny, nz)];
+    const float4 yp = a.mac[cidx(cx, cy + 1, cz, nx, ny, nz)], ym = a.mac[cidx(cx, cy - 1, cz, nx, ny, nz)];
+    const float4 zp = a.mac[cidx(cx, cy, cz + 1, nx, ny, nz)], zm = a.mac[cidx(cx, cy, cz - 1, nx, ny, nz)];
+    const float dxy = (xp.y - xm.y) * 0.5f, dxz = (xp.z - xm.z) * 0.5f;
+    const float dyx = (yp.x - ym.x) * 0.5f, dyz = (yp.z - ym.z) * 0.5f;
+    const float dzx = (zp.x - zm.x) * 0.5f, dzy = (zp.y - zm.y) * 0.5f;
+    const float wx = dyz - dzy, wy = dzx - dxz, wz = dxy - dyx;
+    v = sqrtf(wx * wx + wy * wy + wz * wz) * a.lref / a.uin;
+  }
+  const float t = clampf((v - a.vmin) / (a.vmax - a.vmin), 0.0f, 1.0f);
+  uchar2 o;
+  o.x = (uint8_t)(int)(t * 255.0f + 0.5f);
+  o.y = (a.cls[idx] & CLS_CODE) != 0 ? 0 : 255;
+  reinterpret_cast<uchar2*>(a.out)[k] = o;
+}
+
+// Fluid-only trilinear density, lbm.py:321-350.
+__device__ float2 rho_fluid(const float4* __restrict__ mac, const uint8_t* __restrict__ cls, float px,
+                            float py, float pz, int nx, int ny, int nz) {
+  const float qx = clampf(px, 0.0f, (float)nx - 1.001f);
+  const float qy = clampf(py, 0.0f, (float)ny - 1.001f);
+  const float qz = clampf(pz, 0.0f, (float)nz - 1.001f);
+  const int ix = (int)floorf(qx), iy = (int)floorf(qy), iz = (int)floorf(qz);
+  const float tx = qx - (float)ix, ty = qy - (float)iy, tz = qz - (float)iz;
+  float s0 = 0.0f, s1 = 0.0f;
+#pragma unroll
+  for (int c = 0; c < 8; c++) {
+    const int ox = c & 1, oy = (c >> 1) & 1, oz = (c >> 2) & 1;
+    const uint32_t ci = cidx(ix + ox, iy + oy, iz + oz, nx, ny, nz);
+    const float wx = ox ? tx : 1.0f - tx;
+    const float wy = oy ? ty : 1.0f - ty;
+    const float wz = oz ? tz : 1.0f - tz;
+    const float wt = wx * wy * wz;
+    if ((cls[ci] & CLS_CODE) == 0) {
+      s0 = s0 + wt * mac[ci].w;
+      s1 = s1 + wt;
+    }
+  }
+  return make_float2(s0, s1);
+}
+
+struct CpArgs {
+  const float4* mac;
+  const uint8_t* cls;
+  const float* verts;
+  const float* norms;
+  int8_t* out;
+  uint32_t nv;
+  int nx, ny, nz;
+  float uin, cp_range;
+};
+
+// lbm.py:353-368
+__global__ __launch_bounds__(ABLOCK) void k_cp(CpArgs a) {
+  const uint32_t i = blockIdx.x * ABLOCK + threadIdx.x;
+  if (i >= a.nv) return;
+  const float px = a.verts[3 * (size_t)i], py = a.verts[3 * (size_t)i + 1], pz = a.verts[3 * (size_t)i + 2];
+  const float rx = a.norms[3 * (size_t)i], ry = a.norms[3 * (size_t)i + 1], rz = a.norms[3 * (size_t)i + 2];
+  const float2 A = rho_fluid(a.mac, a.cls, px + rx * 1.25f, py + ry * 1.25f, pz + rz * 1.25f, a.nx, a.ny, a.nz);
+  const float2 B = rho_fluid(a.mac, a.cls, px - rx * 1.25f, py - ry * 1.25f, pz - rz * 1.25f, a.nx, a.ny, a.nz);
+  float2 s = B;
+  if (A.y >= B.y) s = A;
+  float cp = 0.0f;
+  if (s.y > 0.05f) cp = (s.x / s.y - 1.0f) / (1.5f * a.uin * a.uin);
+  a.out[i] = (int8_t)(int)roundf(clampf(cp / a.cp_range, -1.0f, 1.0f) * 127.0f);
+}
+
+int check_small_grid(const wt_grid* g) {
+  if (!g || !g->mac || !g->cls) return fail(-1, "null grid, mac or class buffer");
+  if (g->nx < 3 || g->ny < 3 || g->nz < 3) return fail(-1, "grid dimensions must be at least 3");
+  if ((int64_t)g->nx * g->ny * g->nz >= ((int64_t)1 << 31)) return fail(-1, "grid too large");
+  return 0;
+}
+
+}  // namespace wt
+
+using namespace wt;
+
+extern "C" {
+
+const char* wt_last_error(void) { return error_buffer(); }
+
+int wt_version(void) { return 1; }
+
+int wt_device_info(char* name, int name_len, int64_t* total_bytes) {
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess) return fail(-2, "hipGetDevice failed");
+  hipDeviceProp_t prop;
+  if (hipGetDeviceProperties(&prop, dev) != hipSuccess) return fail(-2, "hipGetDeviceProperties failed");
+  if (name && name_len > 0) {
+    strncpy(name, prop.name, (size_t)name_len - 1);
+    name[name_len - 1] = 0;
+  }
+  if (total_bytes) *total_bytes = (int64_t)prop.totalGlobalMem;
+  return 0;
+}
+
+int wt_advect(const wt_grid* g, float* pos, float* spd, int64_t num_p, float dt, float uin, float max_age,
+              uint32_t seed, const wt_seed_box* box, void* stream) {
+  if (int e = check_small_grid(g)) return e;
+  if (num_p < 0 || num_p >= ((int64_t)1 << 31)) return fail(-1, "bad particle count %lld", (long long)num_p);
+  if (num_p == 0) return 0;
+  if (!pos || !spd || !box) return fail(-1, "null particle buffer or seed box");
+  AdvectArgs a{reinterpret_cast<const float4*>(g->mac), g->cls, reinterpret_cast<float4*>(pos), spd,
+               (uint32_t)num_p, g->nx, g->ny, g->nz, dt, uin, max_age, seed, *box};
+  hipLaunchKernelGGL(k_advect, dim3((unsigned)((num_p + ABLOCK - 1) / ABLOCK)), dim3(ABLOCK), 0,
+                     (hipStream_t)stream, a);
+  return check_launch("wt_advect");
+}
+
+int wt_pack_particles(const wt_grid* g, const float* pos, const float* spd, uint16_t* q16, uint8_t* q8,
+                      int64_t num_p, void* stream) {
+  if (!g) return fail(-1, "null grid");
+  if (num_p < 0 || num_p >= ((int64_t)1 << 31)) return fail(-1, "bad particle count %lld", (long long)num_p);
+  if (num_p == 0) return 0;
+  if (!pos || !spd || !q16 || !q8) return fail(-1, "null particle buffer");
+  hipLaunchKernelGGL(k_pack, dim3((unsigned)((num_p + ABLOCK - 1) / ABLOCK)), dim3(ABLOCK), 0,
+                     (hipStream_t)stream, reinterpret_cast<const float4*>(pos), spd,
+                     reinterpret_cast<ushort4*>(q16), q8, (uint32_t)num_p, g->nx, g->ny, g->nz);
+  return check_launch("wt_pack_particles");
+}
+
+int wt_slice(const wt_grid* g, uint8_t* out, int axis, int index, int field, float uin, float lref, float vmin,
+             float vmax, void* stream) {
+  if (int e = check_small_grid(g)) return e;
+  if (!out) return fail(-1, "null output");
+  if (axis < 0 || axis > 2) return fail(-1, "slice axis must be 0, 1 or 2 (got %d)", axis);
+  const int dims[3] = {g->nx, g->ny, g->nz};
+  if (index < 0 || index >= dims[axis])
+    return fail(-1, "slice index %d outside [0, %d) on axis %d", index, dims[axis], axis);
+  // slice_dims, lbm.py:455-456
+  const int w = axis == 2 ? g->nx : (axis == 1 ? g->nx : g->nz);
+  const int h = axis == 2 ? g->ny : (axis == 1 ? g->nz : g->ny);
+  SliceArgs a{reinterpret_cast<const float4*>(g->mac), g->cls, out, g->nx, g->ny, g->nz, axis, index, field,
+              w, (uint32_t)w * (uint32_t)h, uin, lref, vmin, vmax};
+  hipLaunchKernelGGL(k_slice, dim3((a.npix + ABLOCK - 1) / ABLOCK), dim3(ABLOCK), 0, (hipStream_t)stream, a);
+  return check_launch("wt_slice");
+}
+
+int wt_cp(const wt_grid* g, const float* verts, const float* norms, int8_t* out, int64_t nv, float uin,
+          float cp_range, void* stream) {
+  if (int e = check_small_grid(g)) return e;
+  if (nv < 0 || nv >= ((int64_t)1 << 31)) return fail(-1, "bad vertex count %lld", (long long)nv);
+  if (nv == 0) return 0;
+  if (!verts || !norms || !out) return fail(-1, "null vertex, normal or output buffer");
+  CpArgs a{reinterpret_cast<const float4*>(g->mac), g->cls, verts, norms, out, (uint32_t)nv,
+           g->nx, g->ny, g->nz, uin, cp_range};
+  hipLaunchKernelGGL(k_cp, dim3((unsigned)((nv + ABLOCK - 1) / ABLOCK)), dim3(ABLOCK), 0, (hipStream_t)stream, a);
+  return check_launch("wt_cp");
+}
+
+}  // extern "C"

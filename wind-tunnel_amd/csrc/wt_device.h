@@ -1,0 +1,129 @@
+// Device-side building blocks of the D3Q19 solver for gfx950: lattice tables,
+// population storage forms, grid descriptors and integer helpers.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "wt.h"
+
+namespace wt {
+
+constexpr int Q = 19;
+
+// D3Q19 velocity set in the reference's order (lbm.py:15-17): 0 rest,
+// 1-6 axis +-x, +-y, +-z, 7-18 diagonals in opposite pairs.
+constexpr int cx(int i) {
+  constexpr int t[Q] = {0, 1, -1, 0, 0, 0, 0, 1, -1, 1, -1, 1, -1, 1, -1, 0, 0, 0, 0};
+  return t[i];
+}
+constexpr int cy(int i) {
+  constexpr int t[Q] = {0, 0, 0, 1, -1, 0, 0, 1, -1, -1, 1, 0, 0, 0, 0, 1, -1, 1, -1};
+  return t[i];
+}
+constexpr int cz(int i) {
+  constexpr int t[Q] = {0, 0, 0, 0, 0, 1, -1, 0, 0, 0, 0, 1, -1, -1, 1, 1, -1, -1, 1};
+  return t[i];
+}
+// opposite direction, lbm.py:19
+constexpr int opp(int i) {
+  constexpr int t[Q] = {0, 2, 1, 4, 3, 6, 5, 8, 7, 10, 9, 12, 11, 14, 13, 16, 15, 18, 17};
+  return t[i];
+}
+// lattice weight in double, lbm.py:18
+constexpr double wd(int i) { return i == 0 ? 1.0 / 3.0 : (i <= 6 ? 1.0 / 18.0 : 1.0 / 36.0); }
+
+// class byte written by wt_set_solid
+constexpr uint8_t CLS_CODE = 3;   // 0 fluid, 1 static solid, 2 belt, 3 other solid
+constexpr uint8_t CLS_NEAR = 128; // interior fluid cell with a non-fluid neighbour
+constexpr uint8_t CLS_FORCE = 64; // interior fluid cell with a static-solid neighbour
+
+// ---- storage forms ----------------------------------------------------------
+template <int FORM> struct Form;
+
+template <> struct Form<WT_FP32> {
+  typedef float elem;
+  typedef float real;
+  __device__ static float load(elem e, int) { return e; }
+  __device__ static elem store(float f, int) { return f; }
+};
+
+__device__ inline float h2f(uint16_t b) { return (float)__builtin_bit_cast(_Float16, b); }
+__device__ inline uint16_t f2h(float f) { return __builtin_bit_cast(uint16_t, (_Float16)f); }
+__device__ inline float bf2f(uint16_t b) { return __uint_as_float((uint32_t)b << 16); }
+__device__ inline uint16_t f2bf(float f) {
+  uint32_t x = __float_as_uint(f);
+  if ((x & 0x7fffffffu) > 0x7f800000u) return (uint16_t)((x >> 16) | 0x40u);
+  x += 0x7fffu + ((x >> 16) & 1u);
+  return (uint16_t)(x >> 16);
+}
+
+template <> struct Form<WT_FP16> {
+  typedef uint16_t elem;
+  typedef float real;
+  __device__ static float load(elem e, int i) { return (float)wd(i) + h2f(e); }
+  __device__ static elem store(float f, int i) { return f2h(f - (float)wd(i)); }
+};
+
+template <> struct Form<WT_BF16> {
+  typedef uint16_t elem;
+  typedef float real;
+  __device__ static float load(elem e, int i) { return (float)wd(i) + bf2f(e); }
+  __device__ static elem store(float f, int i) { return f2bf(f - (float)wd(i)); }
+};
+
+template <> struct Form<WT_FP64> {
+  typedef double elem;
+  typedef double real;
+  __device__ static double load(elem e, int) { return e; }
+  __device__ static elem store(double f, int) { return f; }
+};
+
+inline int form_elem_bytes(int form) { return form == WT_FP32 ? 4 : (form == WT_FP64 ? 8 : 2); }
+inline int form_real_bytes(int form) { return form == WT_FP64 ? 8 : 4; }
+
+// ---- exact unsigned division by a runtime constant ---------------------------
+// q = floor(x / d) for every x < 2^31, with m = ceil(2^(31+s) / d) < 2^32 and
+// s = ceil(log2 d) (round-up reciprocal method for 31-bit dividends).
+struct Div {
+  uint64_t m;
+  uint32_t s;
+};
+inline Div make_div(uint32_t d) {
+  uint32_t s = 0;
+  while ((1ull << s) < d) s++;
+  uint64_t num = 1ull << (31 + s);
+  uint64_t m = (num + d - 1) / d;
+  return Div{m, 31 + s};
+}
+__device__ __forceinline__ uint32_t divu(uint32_t x, uint64_t m, uint32_t sh) {
+  return (uint32_t)(((uint64_t)x * m) >> sh);
+}
+
+// ---- per-tunnel descriptor used by the step kernels ----------------------------
+struct StepDesc {
+  const void* fin;
+  void* fout;
+  const uint8_t* cls;
+  float4* mac;
+  const void* shell_in;
+  void* shell_out;
+  double* partials;   // per-tile slots of 4 doubles, or null
+  uint64_t mx, my;    // magic multipliers for / nx and / ny
+  uint32_t sx, sy;
+  int32_t nx, ny, nz;
+  uint32_t n;
+  float uin, tau, smag;
+  uint32_t tile_begin; // first global tile of this tunnel (batched launches)
+  uint32_t ntiles;
+  int32_t flags;
+};
+
+// Shell layout: six layers next to the faces, 4 reals per entry.
+// xlo (x=1) [ny*nz], xhi (x=nx-2) [ny*nz], ylo (y=1) [nx*nz], yhi (y=ny-2) [nx*nz],
+// zlo (z=1) [nx*ny], zhi (z=nz-2) [nx*ny].
+inline int64_t shell_entries(int nx, int ny, int nz) {
+  return 2 * ((int64_t)ny * nz + (int64_t)nx * nz + (int64_t)nx * ny);
+}
+
+}  // namespace wt

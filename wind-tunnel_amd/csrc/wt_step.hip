@@ -1,0 +1,908 @@
+// Fused D3Q19 step for gfx950 and the kernels that share its cell logic.
+//
+// One launch = one lattice step of one tunnel (or of every tunnel of a batch):
+// for each cell, in registers, it performs
+//   face equilibrium boundary conditions (lbm.py:84-109),
+//   pull streaming with bounce-back and the moving belt (lbm.py:110-124),
+//   moments with the velocity clamp (lbm.py:125-136),
+//   regularized collision with Smagorinsky LES and the outlet sponge
+//   (lbm.py:137-165), and the macroscopic store (lbm.py:166),
+// optionally followed by the momentum-exchange force (lbm.py:169-197) and the
+// mass sum of the cells it just updated, reduced in FP64 with wavefront
+// shuffles into fixed slots so the result does not depend on scheduling.
+//
+// Layout: populations are SoA planes (population i of cell idx at i*n + idx).
+// A workgroup is 256 threads = 4 wave64; lanes run along x, so each population
+// load or store of a wave is one contiguous 256 B (FP32) or 128 B (FP16) run.
+// Link classes come from the class byte built once per geometry: only cells
+// flagged as touching a solid gather their 18 neighbour classes.
+//
+// Face cells read the previous step's macroscopic values of their nearest
+// interior cell from a small ping-pong snapshot of the six layers next to the
+// faces (deterministic Jacobi rule; the reference reads mac[] in the same
+// launch that writes it, lbm.py:87 vs lbm.py:166).
+//
+// The whole library is compiled with -ffp-contract=off: each operation is
+// rounded once, in the order the reference's generated source uses, so the
+// FP32 path can be compared bit for bit with the CPU restatement.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstdio>
+#include <vector>
+
+#include "wt_device.h"
+#include "wt_host.h"
+
+namespace wt {
+
+constexpr int BLOCK = 256;       // threads per workgroup (4 wave64)
+constexpr int SLOT = 256;        // cells per reduction slot
+constexpr int CHUNK = 1024;      // slots per first-level reduction block
+
+template <typename T> __device__ __forceinline__ T clampv(T v, T a, T b) {
+  return (v < a) ? a : ((v > b) ? b : v);
+}
+// wp.max(a, b) on floats, with the same choice for equal values as the oracle
+template <typename T> __device__ __forceinline__ T maxv(T a, T b) { return (a < b) ? b : a; }
+
+template <typename E> __device__ __forceinline__ E ld_stream(const E* p) {
+  return __builtin_nontemporal_load(p);
+}
+template <typename E> __device__ __forceinline__ void st_stream(E* p, E v) {
+  __builtin_nontemporal_store(v, p);
+}
+
+// Signed sum c_i . u in index order, as cu_expr() emits it (lbm.py:48-53).
+template <typename R>
+__device__ __forceinline__ R cdot(int i, R ux, R uy, R uz) {
+  R s = (R)0;
+  if (cx(i) > 0) s = s + ux; else if (cx(i) < 0) s = s - ux;
+  if (cy(i) > 0) s = s + uy; else if (cy(i) < 0) s = s - uy;
+  if (cz(i) > 0) s = s + uz; else if (cz(i) < 0) s = s - uz;
+  return s;
+}
+
+// ---- shell (face-adjacent snapshot) ---------------------------------------------
+template <typename R> struct Shell {
+  R* base;
+  int32_t nx, ny, nz;
+  __device__ R* xlo() const { return base; }
+  __device__ R* xhi() const { return base + 4 * (size_t)ny * nz; }
+  __device__ R* ylo() const { return base + 8 * (size_t)ny * nz; }
+  __device__ R* yhi() const { return base + 8 * (size_t)ny * nz + 4 * (size_t)nx * nz; }
+  __device__ R* zlo() const { return base + 8 * ((size_t)ny * nz + (size_t)nx * nz); }
+  __device__ R* zhi() const { return base + 8 * ((size_t)ny * nz + (size_t)nx * nz) + 4 * (size_t)nx * ny; }
+};
+
+template <typename R>
+__device__ __forceinline__ void put4(R* p, R a, R b, R c, R d) {
+  p[0] = a; p[1] = b; p[2] = c; p[3] = d;
+}
+template <>
+__device__ __forceinline__ void put4<float>(float* p, float a, float b, float c, float d) {
+  *reinterpret_cast<float4*>(p) = make_float4(a, b, c, d);
+}
+
+// Interior and solid cells on one of the six layers publish their macroscopic
+// values for the next step's face cells.
+template <typename R>
+__device__ __forceinline__ void shell_write(const Shell<R>& s, int x, int y, int z, R ux, R uy, R uz,
+                                            R rho) {
+  const int nx = s.nx, ny = s.ny, nz = s.nz;
+  if (x == 1) put4(s.xlo() + 4 * (size_t)(y + ny * z), ux, uy, uz, rho);
+  if (x == nx - 2) put4(s.xhi() + 4 * (size_t)(y + ny * z), ux, uy, uz, rho);
+  if (y == 1) put4(s.ylo() + 4 * (size_t)(x + nx * z), ux, uy, uz, rho);
+  if (y == ny - 2) put4(s.yhi() + 4 * (size_t)(x + nx * z), ux, uy, uz, rho);
+  if (z == 1) put4(s.zlo() + 4 * (size_t)(x + nx * y), ux, uy, uz, rho);
+  if (z == nz - 2) put4(s.zhi() + 4 * (size_t)(x + nx * y), ux, uy, uz, rho);
+}
+
+// The previous-step values of a face cell's nearest interior cell
+// clamp(x,1,nx-2), clamp(y,1,ny-2), clamp(z,1,nz-2) (lbm.py:87). Never called
+// for x == 0, whose inlet values do not depend on it (lbm.py:96-100).
+template <typename R>
+__device__ __forceinline__ const R* shell_read(const Shell<R>& s, int x, int y, int z) {
+  const int nx = s.nx, ny = s.ny, nz = s.nz;
+  const int tx = clampv(x, 1, nx - 2), ty = clampv(y, 1, ny - 2), tz = clampv(z, 1, nz - 2);
+  if (x == nx - 1) return s.xhi() + 4 * (size_t)(ty + ny * tz);
+  if (y == 0) return s.ylo() + 4 * (size_t)(tx + nx * tz);
+  if (y == ny - 1) return s.yhi() + 4 * (size_t)(tx + nx * tz);
+  if (z == 0) return s.zlo() + 4 * (size_t)(tx + nx * ty);
+  return s.zhi() + 4 * (size_t)(tx + nx * ty);
+}
+
+// ---- link classification ------------------------------------------------------
+// Bounce-back and belt bitmasks of an interior fluid cell from the classes of
+// its 18 upstream neighbours idx - c_i (lbm.py:114-124).
+struct Links {
+  uint32_t bb, belt;
+};
+__device__ __forceinline__ int32_t offs(int i, int32_t nx, int32_t nxy) {
+  return cx(i) + nx * cy(i) + nxy * cz(i);
+}
+__device__ __forceinline__ Links gather_links(const uint8_t* cls, uint32_t idx, int32_t nx,
+                                              int32_t nxy) {
+  Links l{0u, 0u};
+#pragma unroll
+  for (int i = 1; i < Q; i++) {
+    const uint32_t t = cls[idx - offs(i, nx, nxy)] & CLS_CODE;
+    l.bb |= (t != 0u ? 1u : 0u) << i;
+    if (cx(i) != 0) l.belt |= (t == 2u ? 1u : 0u) << i;
+  }
+  return l;
+}
+// Links i whose downstream neighbour idx + c_i is a static solid (lbm.py:188).
+__device__ __forceinline__ uint32_t gather_force_links(const uint8_t* cls, uint32_t idx, int32_t nx,
+                                                       int32_t nxy) {
+  uint32_t m = 0;
+#pragma unroll
+  for (int i = 1; i < Q; i++) {
+    const uint32_t t = cls[idx + offs(i, nx, nxy)] & CLS_CODE;
+    m |= (t == 1u ? 1u : 0u) << i;
+  }
+  return m;
+}
+
+struct Acc {
+  double fx, fy, fz, m;
+};
+
+// ---- one cell of the fused step ---------------------------------------------------
+template <int FORM>
+__device__ __forceinline__ void step_cell(const StepDesc& d, const typename Form<FORM>::elem* fin,
+                                          typename Form<FORM>::elem* fout,
+                                          const Shell<typename Form<FORM>::real>& shin,
+                                          const Shell<typename Form<FORM>::real>& shout,
+                                          uint32_t idx, int flags, Acc& acc) {
+  typedef Form<FORM> S;
+  typedef typename S::real R;
+  typedef typename S::elem E;
+  const int32_t nx = d.nx, ny = d.ny, nz = d.nz;
+  const uint32_t n = d.n;
+  const int32_t nxy = nx * ny;
+  const uint32_t yz = divu(idx, d.mx, d.sx);
+  const int x = (int)(idx - yz * (uint32_t)nx);
+  const uint32_t zz = divu(yz, d.my, d.sy);
+  const int y = (int)(yz - zz * (uint32_t)ny);
+  const int z = (int)zz;
+  const uint8_t c = d.cls[idx];
+  const R zero = (R)0, one = (R)1;
+
+  if ((c & CLS_CODE) != 0) {  // solid cell, lbm.py:81-83
+    d.mac[idx] = make_float4(0.f, 0.f, 0.f, 1.f);
+    shell_write<R>(shout, x, y, z, zero, zero, zero, one);
+    return;
+  }
+  const bool face = (x == 0) | (x == nx - 1) | (y == 0) | (y == ny - 1) | (z == 0) | (z == nz - 1);
+  if (face) {  // equilibrium boundaries, lbm.py:84-109
+    const R uin = (R)d.uin;
+    R br, bux, buy, buz;
+    if (x == 0) {
+      br = one; bux = uin; buy = zero; buz = zero;
+    } else {
+      const R* m = shell_read<R>(shin, x, y, z);
+      br = clampv(m[3], (R)0.9, (R)1.1);
+      bux = m[0]; buy = m[1]; buz = m[2];
+      if (y == 0 || y == ny - 1) buy = zero;
+      if (z == 0 || z == nz - 1) buz = zero;
+      if (x == nx - 1) { br = one; bux = maxv(bux, zero); }
+    }
+    const R bsq = (R)1.5 * (bux * bux + buy * buy + buz * buz);
+#pragma unroll
+    for (int i = 0; i < Q; i++) {
+      const R bc = (R)3.0 * cdot<R>(i, bux, buy, buz);
+      const R v = (R)wd(i) * br * (one + bc + (R)0.5 * bc * bc - bsq);
+      st_stream(fout + (size_t)i * n + idx, S::store(v, i));
+    }
+    d.mac[idx] = make_float4((float)bux, (float)buy, (float)buz, (float)br);
+    if (flags & WT_FLAG_MASS) acc.m += (double)br;
+    return;
+  }
+
+  // pull streaming with bounce-back and the moving belt, lbm.py:110-124
+  Links lk{0u, 0u};
+  if (c & CLS_NEAR) lk = gather_links(d.cls, idx, nx, nxy);
+  R f[Q];
+#pragma unroll
+  for (int i = 0; i < Q; i++) {
+    if (i == 0) {
+      f[0] = S::load(ld_stream(fin + idx), 0);
+    } else {
+      const bool b = (lk.bb >> i) & 1u;
+      const size_t a = b ? (size_t)opp(i) * n + idx : (size_t)i * n + (idx - offs(i, nx, nxy));
+      f[i] = S::load(ld_stream(fin + a), i);
+    }
+  }
+  if (lk.belt) {
+    const R uin = (R)d.uin;
+#pragma unroll
+    for (int i = 1; i < Q; i++)
+      if (cx(i) != 0 && ((lk.belt >> i) & 1u)) f[i] = f[i] + (R)(6.0 * wd(i) * cx(i)) * uin;
+  }
+
+  // moments and velocity clamp, lbm.py:125-136
+  R rho = zero;
+#pragma unroll
+  for (int i = 0; i < Q; i++) rho = rho + f[i];
+  const R ir = one / rho;
+  R sx = zero, sy = zero, sz = zero;
+#pragma unroll
+  for (int i = 0; i < Q; i++) { if (cx(i) > 0) sx = sx + f[i]; else if (cx(i) < 0) sx = sx - f[i]; }
+#pragma unroll
+  for (int i = 0; i < Q; i++) { if (cy(i) > 0) sy = sy + f[i]; else if (cy(i) < 0) sy = sy - f[i]; }
+#pragma unroll
+  for (int i = 0; i < Q; i++) { if (cz(i) > 0) sz = sz + f[i]; else if (cz(i) < 0) sz = sz - f[i]; }
+  R ux = sx * ir, uy = sy * ir, uz = sz * ir;
+  const R um = sqrt(ux * ux + uy * uy + uz * uz);
+  if (um > (R)0.35) {
+    const R sc = (R)0.35 / um;
+    ux = ux * sc; uy = uy * sc; uz = uz * sc;
+  }
+  const R usq = (R)1.5 * (ux * ux + uy * uy + uz * uz);
+
+  // non-equilibrium second moments, lbm.py:137-146; each sum in index order
+  R pxx = zero, pyy = zero, pzz = zero, pxy = zero, pxz = zero, pyz = zero;
+#pragma unroll
+  for (int i = 0; i < Q; i++) {
+    const R cu = (R)3.0 * cdot<R>(i, ux, uy, uz);
+    const R e = (R)wd(i) * rho * (one + cu + (R)0.5 * cu * cu - usq);
+    const R di = f[i] - e;
+    if (cx(i) != 0) pxx = pxx + di;
+    if (cy(i) != 0) pyy = pyy + di;
+    if (cz(i) != 0) pzz = pzz + di;
+    if (cx(i) * cy(i) > 0) pxy = pxy + di; else if (cx(i) * cy(i) < 0) pxy = pxy - di;
+    if (cx(i) * cz(i) > 0) pxz = pxz + di; else if (cx(i) * cz(i) < 0) pxz = pxz - di;
+    if (cy(i) * cz(i) > 0) pyz = pyz + di; else if (cy(i) * cz(i) < 0) pyz = pyz - di;
+  }
+  // Smagorinsky relaxation time and outlet sponge, lbm.py:147-154
+  const R qn = sqrt(pxx * pxx + pyy * pyy + pzz * pzz + (R)2.0 * (pxy * pxy + pxz * pxz + pyz * pyz));
+  R tau0 = (R)d.tau;
+  const R sp = (R)x - (R)nx + (R)24.0;
+  if (sp > zero) tau0 = tau0 + sp * (R)0.015;
+  const R smag = (R)d.smag;
+  const R tau_e = (R)0.5 * (tau0 + sqrt(tau0 * tau0 + (R)25.4558441 * smag * smag * qn * ir));
+  const R keep = one - one / tau_e;
+  const R tr3 = (pxx + pyy + pzz) * ((R)1.0 / (R)3.0);
+
+  // regularized collision and store, lbm.py:155-166
+  E st[Q];
+#pragma unroll
+  for (int i = 0; i < Q; i++) {
+    R acc2 = zero;
+    bool any = false;
+    if (cx(i) != 0) { acc2 = pxx; any = true; }
+    if (cy(i) != 0) { acc2 = any ? acc2 + pyy : pyy; any = true; }
+    if (cz(i) != 0) { acc2 = any ? acc2 + pzz : pzz; any = true; }
+    if (cx(i) * cy(i) != 0) { const R t = (R)(2 * cx(i) * cy(i)) * pxy; acc2 = any ? acc2 + t : t; any = true; }
+    if (cx(i) * cz(i) != 0) { const R t = (R)(2 * cx(i) * cz(i)) * pxz; acc2 = any ? acc2 + t : t; any = true; }
+    if (cy(i) * cz(i) != 0) { const R t = (R)(2 * cy(i) * cz(i)) * pyz; acc2 = any ? acc2 + t : t; any = true; }
+    const R qp = acc2 - tr3;
+    const R cu = (R)3.0 * cdot<R>(i, ux, uy, uz);
+    const R e = (R)wd(i) * rho * (one + cu + (R)0.5 * cu * cu - usq);
+    const R v = e + (R)(4.5 * wd(i)) * keep * qp;
+    st[i] = S::store(v, i);
+    st_stream(fout + (size_t)i * n + idx, st[i]);
+  }
+  d.mac[idx] = make_float4((float)ux, (float)uy, (float)uz, (float)rho);
+  shell_write<R>(shout, x, y, z, ux, uy, uz, rho);
+  if (flags & WT_FLAG_MASS) acc.m += (double)rho;
+  if ((flags & WT_FLAG_FORCES) && (c & CLS_FORCE)) {
+    // momentum exchange on the populations just stored, lbm.py:185-193
+    const uint32_t fl = gather_force_links(d.cls, idx, nx, nxy);
+#pragma unroll
+    for (int i = 1; i < Q; i++) {
+      if ((fl >> i) & 1u) {
+        const R v = (R)2.0 * (S::load(st[i], i) - (R)wd(i));
+        const double dv = (double)v;
+        if (cx(i) > 0) acc.fx += dv; else if (cx(i) < 0) acc.fx -= dv;
+        if (cy(i) > 0) acc.fy += dv; else if (cy(i) < 0) acc.fy -= dv;
+        if (cz(i) > 0) acc.fz += dv; else if (cz(i) < 0) acc.fz -= dv;
+      }
+    }
+  }
+}
+
+// ---- deterministic FP64 reduction ------------------------------------------------
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+  for (int o = 32; o >= 1; o >>= 1) v += __shfl_xor(v, o, 64);
+  return v;
+}
+
+// Each 256-cell slot is reduced the same way whatever the launch shape: the
+// 64 lanes of each of its 4 waves by the shuffle tree above, then the 4 wave
+// sums in wave order.
+__device__ __forceinline__ void slot_stage(double (*lds)[4][4], int k, Acc a) {
+  const double fx = wave_sum(a.fx), fy = wave_sum(a.fy), fz = wave_sum(a.fz), m = wave_sum(a.m);
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  if (lane == 0) {
+    lds[k][wave][0] = fx; lds[k][wave][1] = fy; lds[k][wave][2] = fz; lds[k][wave][3] = m;
+  }
+}
+
+template <int CPT>
+__device__ __forceinline__ void slot_flush(double (*lds)[4][4], double* partials, uint32_t slot0,
+                                           uint32_t nslots) {
+  __syncthreads();
+  const int t = threadIdx.x;
+  if (t < 4 * CPT) {
+    const int k = t >> 2, comp = t & 3;
+    if (slot0 + k < nslots) {
+      const double s = ((lds[k][0][comp] + lds[k][1][comp]) + lds[k][2][comp]) + lds[k][3][comp];
+      partials[4 * (size_t)(slot0 + k) + comp] = s;
+    }
+  }
+}
+
+// ---- the tile loop ------------------------------------------------------------------
+template <int FORM, int CPT>
+__device__ __forceinline__ void run_tile(const StepDesc& d, int parity, uint32_t tile, int flags) {
+  typedef Form<FORM> S;
+  typedef typename S::real R;
+  typedef typename S::elem E;
+  __shared__ double lds[CPT][4][4];
+  const int side = parity & 1;
+  const E* fin = (const E*)(side ? d.fout : d.fin);
+  E* fout = (E*)(side ? d.fin : d.fout);
+  const Shell<R> shin{(R*)(side ? d.shell_out : d.shell_in), d.nx, d.ny, d.nz};
+  const Shell<R> shout{(R*)(side ? d.shell_in : d.shell_out), d.nx, d.ny, d.nz};
+  const uint32_t base = tile * (uint32_t)(BLOCK * CPT);
+#pragma unroll
+  for (int k = 0; k < CPT; k++) {
+    const uint32_t idx = base + (uint32_t)(k * BLOCK) + threadIdx.x;
+    Acc a{0.0, 0.0, 0.0, 0.0};
+    if (idx < d.n) step_cell<FORM>(d, fin, fout, shin, shout, idx, flags, a);
+    if (flags & (WT_FLAG_FORCES | WT_FLAG_MASS)) slot_stage(lds, k, a);
+  }
+  if (flags & (WT_FLAG_FORCES | WT_FLAG_MASS))
+    slot_flush<CPT>(lds, d.partials, tile * CPT, (d.n + SLOT - 1) / SLOT);
+}
+
+// Workgroups are dispatched to the 8 XCDs round-robin; this maps them so that
+// each XCD walks one contiguous range of tiles (neighbouring tiles share
+// cache lines through the x +- 1 pulls).
+__device__ __forceinline__ uint32_t xcd_tile(uint32_t b, uint32_t nblocks, int remap) {
+  if (!remap) return b;
+  const uint32_t q = nblocks >> 3, r = nblocks & 7u;
+  const uint32_t xcd = b & 7u, k = b >> 3;
+  return xcd * q + (xcd < r ? xcd : r) + k;
+}
+
+template <int FORM, int CPT>
+__global__ __launch_bounds__(BLOCK) void k_step(StepDesc d, int parity, int remap, int flags) {
+  const uint32_t tile = xcd_tile(blockIdx.x, gridDim.x, remap);
+  run_tile<FORM, CPT>(d, parity, tile, flags);
+}
+
+template <int FORM, int CPT>
+__global__ __launch_bounds__(BLOCK) void k_step_batch(const StepDesc* __restrict__ ds, int nd,
+                                                      int parity, int remap, int flags) {
+  const uint32_t tile = xcd_tile(blockIdx.x, gridDim.x, remap);
+  int lo = 0, hi = nd - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (ds[mid].tile_begin <= tile) lo = mid; else hi = mid - 1;
+  }
+  const StepDesc d = ds[lo];
+  run_tile<FORM, CPT>(d, parity ^ (d.flags & 1), tile - d.tile_begin, flags);
+}
+
+// ---- reductions of the slots ---------------------------------------------------------
+struct RedDesc {
+  double* partials;  // nslots slots, then nchunks first-level sums
+  double* sums;      // 4 doubles
+  uint32_t nslots;
+};
+
+__device__ __forceinline__ double block_sum(double v, double* lds) {
+  v = wave_sum(v);
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  __syncthreads();
+  if (lane == 0) lds[wave] = v;
+  __syncthreads();
+  return ((lds[0] + lds[1]) + lds[2]) + lds[3];
+}
+
+// First level: block (c, t) sums slots [c*CHUNK, (c+1)*CHUNK) of tunnel t.
+__global__ __launch_bounds__(BLOCK) void k_reduce1(const RedDesc* __restrict__ rd) {
+  __shared__ double lds[4];
+  const RedDesc r = rd[blockIdx.y];
+  const uint32_t nchunks = (r.nslots + CHUNK - 1) / CHUNK;
+  if (blockIdx.x >= nchunks) return;
+  const uint32_t s0 = blockIdx.x * CHUNK + threadIdx.x * (CHUNK / BLOCK);
+  double v[4] = {0.0, 0.0, 0.0, 0.0};
+  for (int j = 0; j < CHUNK / BLOCK; j++) {
+    const uint32_t s = s0 + j;
+    if (s < r.nslots) {
+#pragma unroll
+      for (int c = 0; c < 4; c++) v[c] += r.partials[4 * (size_t)s + c];
+    }
+  }
+#pragma unroll
+  for (int c = 0; c < 4; c++) {
+    const double t = block_sum(v[c], lds);
+    if (threadIdx.x == 0) r.partials[4 * ((size_t)r.nslots + blockIdx.x) + c] = t;
+  }
+}
+
+// Second level: block t sums its first-level chunks in order.
+__global__ __launch_bounds__(BLOCK) void k_reduce2(const RedDesc* __restrict__ rd) {
+  __shared__ double lds[4];
+  const RedDesc r = rd[blockIdx.x];
+  const uint32_t nchunks = (r.nslots + CHUNK - 1) / CHUNK;
+  double v[4] = {0.0, 0.0, 0.0, 0.0};
+  for (uint32_t ch = threadIdx.x; ch < nchunks; ch += BLOCK) {
+#pragma unroll
+    for (int c = 0; c < 4; c++) v[c] += r.partials[4 * ((size_t)r.nslots + ch) + c];
+  }
+#pragma unroll
+  for (int c = 0; c < 4; c++) {
+    const double t = block_sum(v[c], lds);
+    if (threadIdx.x == 0) r.sums[c] = t;
+  }
+}
+
+// Single-tunnel variants take the descriptor by value.
+__global__ __launch_bounds__(BLOCK) void k_reduce1_one(RedDesc r) {
+  __shared__ double lds[4];
+  const uint32_t s0 = blockIdx.x * CHUNK + threadIdx.x * (CHUNK / BLOCK);
+  double v[4] = {0.0, 0.0, 0.0, 0.0};
+  for (int j = 0; j < CHUNK / BLOCK; j++) {
+    const uint32_t s = s0 + j;
+    if (s < r.nslots) {
+#pragma unroll
+      for (int c = 0; c < 4; c++) v[c] += r.partials[4 * (size_t)s + c];
+    }
+  }
+#pragma unroll
+  for (int c = 0; c < 4; c++) {
+    const double t = block_sum(v[c], lds);
+    if (threadIdx.x == 0) r.partials[4 * ((size_t)r.nslots + blockIdx.x) + c] = t;
+  }
+}
+__global__ __launch_bounds__(BLOCK) void k_reduce2_one(RedDesc r) {
+  __shared__ double lds[4];
+  const uint32_t nchunks = (r.nslots + CHUNK - 1) / CHUNK;
+  double v[4] = {0.0, 0.0, 0.0, 0.0};
+  for (uint32_t ch = threadIdx.x; ch < nchunks; ch += BLOCK) {
+#pragma unroll
+    for (int c = 0; c < 4; c++) v[c] += r.partials[4 * ((size_t)r.nslots + ch) + c];
+  }
+#pragma unroll
+  for (int c = 0; c < 4; c++) {
+    const double t = block_sum(v[c], lds);
+    if (threadIdx.x == 0) r.sums[c] = t;
+  }
+}
+
+// ---- standalone force on f[cur] (lbm.py:169-197) ---------------------------------------
+template <int FORM>
+__global__ __launch_bounds__(BLOCK) void k_force(StepDesc d) {
+  typedef Form<FORM> S;
+  typedef typename S::real R;
+  typedef typename S::elem E;
+  __shared__ double lds[1][4][4];
+  const uint32_t idx = blockIdx.x * BLOCK + threadIdx.x;
+  Acc a{0.0, 0.0, 0.0, 0.0};
+  if (idx < d.n) {
+    const uint8_t c = d.cls[idx];
+    if (c & CLS_FORCE) {
+      const E* f = (const E*)d.fin;
+      const int32_t nx = d.nx, nxy = d.nx * d.ny;
+      const uint32_t fl = gather_force_links(d.cls, idx, nx, nxy);
+#pragma unroll
+      for (int i = 1; i < Q; i++) {
+        if ((fl >> i) & 1u) {
+          const R v = (R)2.0 * (S::load(f[(size_t)i * d.n + idx], i) - (R)wd(i));
+          const double dv = (double)v;
+          if (cx(i) > 0) a.fx += dv; else if (cx(i) < 0) a.fx -= dv;
+          if (cy(i) > 0) a.fy += dv; else if (cy(i) < 0) a.fy -= dv;
+          if (cz(i) > 0) a.fz += dv; else if (cz(i) < 0) a.fz -= dv;
+        }
+      }
+    }
+  }
+  slot_stage(lds, 0, a);
+  slot_flush<1>(lds, d.partials, blockIdx.x, (d.n + SLOT - 1) / SLOT);
+}
+
+// ---- init (lbm.py:64-69) ----------------------------------------------------------------
+template <int FORM>
+__global__ __launch_bounds__(BLOCK) void k_init(typename Form<FORM>::elem* f0,
+                                                typename Form<FORM>::elem* f1, float4* mac,
+                                                uint32_t n, float uin_f) {
+  typedef Form<FORM> S;
+  typedef typename S::real R;
+  const uint32_t idx = blockIdx.x * BLOCK + threadIdx.x;
+  if (idx >= n) return;
+  const R uin = (R)uin_f;
+#pragma unroll
+  for (int i = 0; i < Q; i++) {
+    const R v = (R)wd(i) * ((R)1.0 + (R)(3 * cx(i)) * uin + (R)(4.5 * cx(i) * cx(i)) * uin * uin -
+                            (R)1.5 * uin * uin);
+    const typename S::elem e = S::store(v, i);
+    f0[(size_t)i * n + idx] = e;
+    f1[(size_t)i * n + idx] = e;
+  }
+  mac[idx] = make_float4(uin_f, 0.f, 0.f, 1.f);
+}
+
+template <typename R>
+__global__ __launch_bounds__(BLOCK) void k_init_shell(R* s0, R* s1, uint32_t entries, float uin) {
+  const uint32_t e = blockIdx.x * BLOCK + threadIdx.x;
+  if (e >= entries) return;
+  put4<R>(s0 + 4 * (size_t)e, (R)uin, (R)0, (R)0, (R)1);
+  put4<R>(s1 + 4 * (size_t)e, (R)uin, (R)0, (R)0, (R)1);
+}
+
+// ---- class bytes from the solid mask (lbm.py:419-421 input) --------------------------------
+__global__ __launch_bounds__(BLOCK) void k_build_cls(const uint8_t* __restrict__ solid, uint8_t* cls,
+                                                     int32_t nx, int32_t ny, int32_t nz, uint32_t n,
+                                                     uint64_t mx, uint32_t sx, uint64_t my,
+                                                     uint32_t sy) {
+  const uint32_t idx = blockIdx.x * BLOCK + threadIdx.x;
+  if (idx >= n) return;
+  const uint32_t yz = divu(idx, mx, sx);
+  const int x = (int)(idx - yz * (uint32_t)nx);
+  const uint32_t zz = divu(yz, my, sy);
+  const int y = (int)(yz - zz * (uint32_t)ny);
+  const int z = (int)zz;
+  auto code = [](uint8_t v) -> uint32_t { return v == 0 ? 0u : (v == 1 ? 1u : (v == 2 ? 2u : 3u)); };
+  const uint32_t own = code(solid[idx]);
+  uint32_t out = own;
+  const bool face = (x == 0) | (x == nx - 1) | (y == 0) | (y == ny - 1) | (z == 0) | (z == nz - 1);
+  if (own == 0 && !face) {
+    const int32_t nxy = nx * ny;
+    bool near = false, force = false;
+#pragma unroll
+    for (int i = 1; i < Q; i++) {
+      const uint32_t t = code(solid[idx + offs(i, nx, nxy)]);
+      near |= (t != 0);
+      force |= (t == 1);
+    }
+    if (near) out |= CLS_NEAR;
+    if (force) out |= CLS_FORCE;
+  }
+  cls[idx] = (uint8_t)out;
+}
+
+// ---- link classes as the step kernel uses them --------------------------------------------
+__global__ __launch_bounds__(BLOCK) void k_link_classes(const uint8_t* __restrict__ cls, uint64_t* out,
+                                                        int32_t nx, int32_t ny, int32_t nz, uint32_t n,
+                                                        uint64_t mx, uint32_t sx, uint64_t my,
+                                                        uint32_t sy) {
+  const uint32_t idx = blockIdx.x * BLOCK + threadIdx.x;
+  if (idx >= n) return;
+  const uint32_t yz = divu(idx, mx, sx);
+  const int x = (int)(idx - yz * (uint32_t)nx);
+  const uint32_t zz = divu(yz, my, sy);
+  const int y = (int)(yz - zz * (uint32_t)ny);
+  const int z = (int)zz;
+  const uint8_t c = cls[idx];
+  uint64_t v;
+  if ((c & CLS_CODE) != 0) {
+    v = 1;
+  } else if ((x == 0) | (x == nx - 1) | (y == 0) | (y == ny - 1) | (z == 0) | (z == nz - 1)) {
+    v = 2;
+  } else {
+    const int32_t nxy = nx * ny;
+    Links lk{0u, 0u};
+    if (c & CLS_NEAR) lk = gather_links(cls, idx, nx, nxy);
+    const uint32_t fl = (c & CLS_FORCE) ? gather_force_links(cls, idx, nx, nxy) : 0u;
+    v = ((uint64_t)lk.bb << 8) | ((uint64_t)lk.belt << 27) | ((uint64_t)fl << 45);
+  }
+  out[idx] = v;
+}
+
+// ================================ host side =========================================
+
+static int64_t nslots_of(int64_t n) { return (n + SLOT - 1) / SLOT; }
+static int64_t nchunks_of(int64_t n) { return (nslots_of(n) + CHUNK - 1) / CHUNK; }
+
+StepDesc make_desc(const wt_grid* g, int cur, const wt_params* p, int flags) {
+  StepDesc d{};
+  d.fin = g->f[cur & 1];
+  d.fout = g->f[(cur & 1) ^ 1];
+  d.cls = g->cls;
+  d.mac = reinterpret_cast<float4*>(g->mac);
+  d.shell_in = g->shell[cur & 1];
+  d.shell_out = g->shell[(cur & 1) ^ 1];
+  d.partials = g->partials;
+  const Div dx = make_div((uint32_t)g->nx), dy = make_div((uint32_t)g->ny);
+  d.mx = dx.m; d.sx = dx.s; d.my = dy.m; d.sy = dy.s;
+  d.nx = g->nx; d.ny = g->ny; d.nz = g->nz;
+  d.n = (uint32_t)((int64_t)g->nx * g->ny * g->nz);
+  if (p) { d.uin = p->uin; d.tau = p->tau; d.smag = p->smag; }
+  d.tile_begin = 0;
+  d.ntiles = 0;
+  d.flags = flags;
+  return d;
+}
+
+int check_grid(const wt_grid* g, bool need_state) {
+  if (!g) return fail(-1, "null grid");
+  if (g->nx < 3 || g->ny < 3 || g->nz < 3)
+    return fail(-1, "grid dimensions must be at least 3 (got %d x %d x %d)", g->nx, g->ny, g->nz);
+  const int64_t n = (int64_t)g->nx * g->ny * g->nz;
+  if (n >= (int64_t)1 << 31) return fail(-1, "grid of %lld cells exceeds 2^31 - 1", (long long)n);
+  if ((int64_t)19 * n >= ((int64_t)1 << 32))
+    return fail(-1, "grid of %lld cells: 19*n population elements exceed 2^32", (long long)n);
+  if (g->form < WT_FP32 || g->form > WT_FP64) return fail(-1, "unknown storage form %d", g->form);
+  if (need_state && (!g->f[0] || !g->f[1] || !g->cls || !g->mac || !g->shell[0] || !g->shell[1]))
+    return fail(-1, "grid has a null device buffer");
+  return 0;
+}
+
+static int reduce_one(const wt_grid* g, hipStream_t s) {
+  const int64_t n = (int64_t)g->nx * g->ny * g->nz;
+  if (!g->partials || !g->sums) return fail(-1, "grid has no reduction buffers");
+  RedDesc r{g->partials, g->sums, (uint32_t)nslots_of(n)};
+  hipLaunchKernelGGL(k_reduce1_one, dim3((unsigned)nchunks_of(n)), dim3(BLOCK), 0, s, r);
+  hipLaunchKernelGGL(k_reduce2_one, dim3(1), dim3(BLOCK), 0, s, r);
+  return check_launch("reduce");
+}
+
+// Launch-shape choice for one storage form: cells per thread and XCD tile order.
+struct Shape {
+  int cpt;
+  int remap;
+};
+static Shape g_shape[4] = {{1, 1}, {1, 1}, {1, 1}, {1, 1}};
+
+template <int FORM, int CPT>
+static void launch_step_t(const StepDesc& d, int parity, int remap, int flags, hipStream_t s) {
+  const uint32_t ntile = (d.n + BLOCK * CPT - 1) / (BLOCK * CPT);
+  hipLaunchKernelGGL((k_step<FORM, CPT>), dim3(ntile), dim3(BLOCK), 0, s, d, parity, remap, flags);
+}
+
+template <int FORM>
+static void launch_step_f(const StepDesc& d, int parity, Shape sh, int flags, hipStream_t s) {
+  switch (sh.cpt) {
+    case 2: launch_step_t<FORM, 2>(d, parity, sh.remap, flags, s); break;
+    case 4: launch_step_t<FORM, 4>(d, parity, sh.remap, flags, s); break;
+    default: launch_step_t<FORM, 1>(d, parity, sh.remap, flags, s); break;
+  }
+}
+
+static void launch_step(int form, const StepDesc& d, int parity, Shape sh, int flags, hipStream_t s) {
+  switch (form) {
+    case WT_FP32: launch_step_f<WT_FP32>(d, parity, sh, flags, s); break;
+    case WT_FP16: launch_step_f<WT_FP16>(d, parity, sh, flags, s); break;
+    case WT_BF16: launch_step_f<WT_BF16>(d, parity, sh, flags, s); break;
+    case WT_FP64: launch_step_f<WT_FP64>(d, parity, sh, flags, s); break;
+  }
+}
+
+template <int FORM, int CPT>
+static void launch_batch_t(const StepDesc* dd, int nd, uint32_t ntile, int parity, int remap,
+                           int flags, hipStream_t s) {
+  hipLaunchKernelGGL((k_step_batch<FORM, CPT>), dim3(ntile), dim3(BLOCK), 0, s, dd, nd, parity, remap,
+                     flags);
+}
+template <int FORM>
+static void launch_batch_f(const StepDesc* dd, int nd, uint32_t ntile, int parity, int cpt, int remap,
+                           int flags, hipStream_t s) {
+  switch (cpt) {
+    case 2: launch_batch_t<FORM, 2>(dd, nd, ntile, parity, remap, flags, s); break;
+    case 4: launch_batch_t<FORM, 4>(dd, nd, ntile, parity, remap, flags, s); break;
+    default: launch_batch_t<FORM, 1>(dd, nd, ntile, parity, remap, flags, s); break;
+  }
+}
+
+}  // namespace wt
+
+using namespace wt;
+
+extern "C" {
+
+int wt_set_launch_shape(int form, int cpt, int remap) {
+  if (form < WT_FP32 || form > WT_FP64) return fail(-1, "unknown storage form %d", form);
+  if (cpt != 1 && cpt != 2 && cpt != 4) return fail(-1, "cells per thread must be 1, 2 or 4 (got %d)", cpt);
+  g_shape[form] = Shape{cpt, remap ? 1 : 0};
+  return 0;
+}
+
+int wt_get_launch_shape(int form, int* cpt, int* remap) {
+  if (form < WT_FP32 || form > WT_FP64) return fail(-1, "unknown storage form %d", form);
+  if (cpt) *cpt = g_shape[form].cpt;
+  if (remap) *remap = g_shape[form].remap;
+  return 0;
+}
+
+int wt_buffer_sizes(int nx, int ny, int nz, int form, int64_t sizes[6]) {
+  if (nx < 3 || ny < 3 || nz < 3) return fail(-1, "grid dimensions must be at least 3");
+  if (form < WT_FP32 || form > WT_FP64) return fail(-1, "unknown storage form %d", form);
+  const int64_t n = (int64_t)nx * ny * nz;
+  sizes[0] = 19 * n * form_elem_bytes(form);
+  sizes[1] = n;
+  sizes[2] = 16 * n;
+  sizes[3] = 4 * shell_entries(nx, ny, nz) * form_real_bytes(form);
+  sizes[4] = 32 * (nslots_of(n) + nchunks_of(n));
+  sizes[5] = 32;
+  return 0;
+}
+
+int64_t wt_state_bytes(int nx, int ny, int nz, int form) {
+  int64_t s[6];
+  if (wt_buffer_sizes(nx, ny, nz, form, s) != 0) return -1;
+  return 2 * s[0] + s[1] + s[2] + 2 * s[3] + s[4] + s[5];
+}
+
+int64_t wt_tile_count(int nx, int ny, int nz) { return nslots_of((int64_t)nx * ny * nz); }
+
+int wt_init(const wt_grid* g, float uin, void* stream) {
+  if (int e = check_grid(g, true)) return e;
+  hipStream_t s = (hipStream_t)stream;
+  const uint32_t n = (uint32_t)((int64_t)g->nx * g->ny * g->nz);
+  const dim3 grid((n + BLOCK - 1) / BLOCK);
+  float4* mac = reinterpret_cast<float4*>(g->mac);
+  const uint32_t ent = (uint32_t)shell_entries(g->nx, g->ny, g->nz);
+  const dim3 sgrid((ent + BLOCK - 1) / BLOCK);
+  switch (g->form) {
+    case WT_FP32:
+      hipLaunchKernelGGL(k_init<WT_FP32>, grid, dim3(BLOCK), 0, s, (float*)g->f[0], (float*)g->f[1], mac, n, uin);
+      break;
+    case WT_FP16:
+      hipLaunchKernelGGL(k_init<WT_FP16>, grid, dim3(BLOCK), 0, s, (uint16_t*)g->f[0], (uint16_t*)g->f[1], mac, n, uin);
+      break;
+    case WT_BF16:
+      hipLaunchKernelGGL(k_init<WT_BF16>, grid, dim3(BLOCK), 0, s, (uint16_t*)g->f[0], (uint16_t*)g->f[1], mac, n, uin);
+      break;
+    case WT_FP64:
+      hipLaunchKernelGGL(k_init<WT_FP64>, grid, dim3(BLOCK), 0, s, (double*)g->f[0], (double*)g->f[1], mac, n, uin);
+      break;
+  }
+  if (g->form == WT_FP64)
+    hipLaunchKernelGGL(k_init_shell<double>, sgrid, dim3(BLOCK), 0, s, (double*)g->shell[0], (double*)g->shell[1], ent, uin);
+  else
+    hipLaunchKernelGGL(k_init_shell<float>, sgrid, dim3(BLOCK), 0, s, (float*)g->shell[0], (float*)g->shell[1], ent, uin);
+  return check_launch("wt_init");
+}
+
+int wt_set_solid(const wt_grid* g, const uint8_t* solid_dev, void* stream) {
+  if (int e = check_grid(g, false)) return e;
+  if (!solid_dev || !g->cls) return fail(-1, "null solid mask or class buffer");
+  const uint32_t n = (uint32_t)((int64_t)g->nx * g->ny * g->nz);
+  const Div dx = make_div((uint32_t)g->nx), dy = make_div((uint32_t)g->ny);
+  hipLaunchKernelGGL(k_build_cls, dim3((n + BLOCK - 1) / BLOCK), dim3(BLOCK), 0, (hipStream_t)stream,
+                     solid_dev, g->cls, g->nx, g->ny, g->nz, n, dx.m, dx.s, dy.m, dy.s);
+  return check_launch("wt_set_solid");
+}
+
+int wt_step(const wt_grid* g, int cur, int nsteps, const wt_params* p, int flags, void* stream) {
+  if (int e = check_grid(g, true)) return e;
+  if (!p) return fail(-1, "null params");
+  if (nsteps < 0) return fail(-1, "negative step count %d", nsteps);
+  if ((flags & (WT_FLAG_FORCES | WT_FLAG_MASS)) && (!g->partials || !g->sums))
+    return fail(-1, "reduction flags need partials and sums buffers");
+  hipStream_t s = (hipStream_t)stream;
+  const Shape sh = g_shape[g->form];
+  const StepDesc d = make_desc(g, cur, p, 0);
+  for (int k = 0; k < nsteps; k++) {
+    const int f = (k == nsteps - 1) ? (flags & (WT_FLAG_FORCES | WT_FLAG_MASS)) : 0;
+    launch_step(g->form, d, k & 1, sh, f, s);
+  }
+  if (nsteps > 0 && (flags & (WT_FLAG_FORCES | WT_FLAG_MASS))) {
+    if (int e = check_launch("wt_step")) return e;
+    return reduce_one(g, s);
+  }
+  return check_launch("wt_step");
+}
+
+int wt_forces(const wt_grid* g, int cur, void* stream) {
+  if (int e = check_grid(g, true)) return e;
+  if (!g->partials || !g->sums) return fail(-1, "grid has no reduction buffers");
+  hipStream_t s = (hipStream_t)stream;
+  StepDesc d = make_desc(g, cur, nullptr, 0);
+  const dim3 grid((d.n + BLOCK - 1) / BLOCK);
+  switch (g->form) {
+    case WT_FP32: hipLaunchKernelGGL(k_force<WT_FP32>, grid, dim3(BLOCK), 0, s, d); break;
+    case WT_FP16: hipLaunchKernelGGL(k_force<WT_FP16>, grid, dim3(BLOCK), 0, s, d); break;
+    case WT_BF16: hipLaunchKernelGGL(k_force<WT_BF16>, grid, dim3(BLOCK), 0, s, d); break;
+    case WT_FP64: hipLaunchKernelGGL(k_force<WT_FP64>, grid, dim3(BLOCK), 0, s, d); break;
+  }
+  if (int e = check_launch("wt_forces")) return e;
+  return reduce_one(g, s);
+}
+
+int wt_read_sums(const wt_grid* g, double out[4], void* stream) {
+  if (!g || !g->sums || !out) return fail(-1, "null grid, sums or output");
+  hipStream_t s = (hipStream_t)stream;
+  if (hipMemcpyAsync(out, g->sums, 4 * sizeof(double), hipMemcpyDeviceToHost, s) != hipSuccess)
+    return fail(-2, "hipMemcpyAsync of sums failed: %s", hipGetErrorString(hipGetLastError()));
+  if (hipStreamSynchronize(s) != hipSuccess)
+    return fail(-2, "stream synchronize failed: %s", hipGetErrorString(hipGetLastError()));
+  return 0;
+}
+
+int wt_link_classes(const wt_grid* g, uint64_t* out_dev, void* stream) {
+  if (int e = check_grid(g, false)) return e;
+  if (!out_dev || !g->cls) return fail(-1, "null output or class buffer");
+  const uint32_t n = (uint32_t)((int64_t)g->nx * g->ny * g->nz);
+  const Div dx = make_div((uint32_t)g->nx), dy = make_div((uint32_t)g->ny);
+  hipLaunchKernelGGL(k_link_classes, dim3((n + BLOCK - 1) / BLOCK), dim3(BLOCK), 0, (hipStream_t)stream,
+                     g->cls, out_dev, g->nx, g->ny, g->nz, n, dx.m, dx.s, dy.m, dy.s);
+  return check_launch("wt_link_classes");
+}
+
+// ---- batched tunnels ---------------------------------------------------------------------
+// Workspace layout: [StepDesc x ntunnels][RedDesc x ntunnels].
+int64_t wt_batch_workspace_bytes(int ntunnels) {
+  if (ntunnels <= 0) {
+    fail(-1, "ntunnels must be positive");
+    return -1;
+  }
+  return (int64_t)ntunnels * (int64_t)(sizeof(StepDesc) + sizeof(RedDesc));
+}
+
+int wt_batch_prepare(const wt_grid* grids, const wt_params* params, const int* cur, int ntunnels,
+                     void* workspace, wt_batch_info* info, void* stream) {
+  if (!grids || !params || !cur || !workspace || !info || ntunnels <= 0)
+    return fail(-1, "null argument or empty batch");
+  const int form = grids[0].form;
+  if (form < WT_FP32 || form > WT_FP64) return fail(-1, "unknown storage form %d", form);
+  const Shape sh = g_shape[form];
+  std::vector<StepDesc> ds(ntunnels);
+  std::vector<RedDesc> rs(ntunnels);
+  uint64_t tile = 0;
+  int64_t max_chunks = 0;
+  for (int k = 0; k < ntunnels; k++) {
+    if (int e = check_grid(&grids[k], true)) return e;
+    if (grids[k].form != form)
+      return fail(-1, "tunnel %d has storage form %d, batch uses %d", k, grids[k].form, form);
+    if (!grids[k].partials || !grids[k].sums) return fail(-1, "tunnel %d has no reduction buffers", k);
+    StepDesc d = make_desc(&grids[k], 0, &params[k], 0);
+    d.tile_begin = (uint32_t)tile;
+    d.ntiles = (d.n + BLOCK * sh.cpt - 1) / (BLOCK * sh.cpt);
+    d.flags = cur[k] & 1;
+    tile += d.ntiles;
+    if (tile >= (1ull << 31)) return fail(-1, "batch has too many tiles");
+    ds[k] = d;
+    rs[k] = RedDesc{grids[k].partials, grids[k].sums, (uint32_t)nslots_of(d.n)};
+    max_chunks = std::max<int64_t>(max_chunks, nchunks_of(d.n));
+  }
+  hipStream_t s = (hipStream_t)stream;
+  char* ws = (char*)workspace;
+  if (hipMemcpyAsync(ws, ds.data(), sizeof(StepDesc) * ntunnels, hipMemcpyHostToDevice, s) != hipSuccess ||
+      hipMemcpyAsync(ws + sizeof(StepDesc) * ntunnels, rs.data(), sizeof(RedDesc) * ntunnels,
+                     hipMemcpyHostToDevice, s) != hipSuccess)
+    return fail(-2, "descriptor upload failed: %s", hipGetErrorString(hipGetLastError()));
+  if (hipStreamSynchronize(s) != hipSuccess)
+    return fail(-2, "stream synchronize failed: %s", hipGetErrorString(hipGetLastError()));
+  info->total_tiles = (int64_t)tile;
+  info->max_chunks = (int32_t)max_chunks;
+  info->form = form;
+  info->ntunnels = ntunnels;
+  info->cpt = sh.cpt;
+  info->remap = sh.remap;
+  return 0;
+}
+
+// One step of every tunnel prepared in `workspace`: with parity p each tunnel
+// k advances from its buffer cur[k] ^ p. Kernel launches only, so a caller
+// may capture it in a graph (two calls, p = 0 and p = 1, per replay).
+int wt_batch_step(void* workspace, const wt_batch_info* info, int parity, int flags, void* stream) {
+  if (!workspace || !info || info->ntunnels <= 0 || info->total_tiles <= 0) return fail(-1, "empty batch");
+  const int form = info->form;
+  if (form < WT_FP32 || form > WT_FP64) return fail(-1, "unknown storage form %d", form);
+  hipStream_t s = (hipStream_t)stream;
+  const int nt = info->ntunnels;
+  const StepDesc* dd = (const StepDesc*)workspace;
+  const RedDesc* rd = (const RedDesc*)((const char*)workspace + sizeof(StepDesc) * nt);
+  const int f = flags & (WT_FLAG_FORCES | WT_FLAG_MASS);
+  const uint32_t ntile = (uint32_t)info->total_tiles;
+  switch (form) {
+    case WT_FP32: launch_batch_f<WT_FP32>(dd, nt, ntile, parity, info->cpt, info->remap, f, s); break;
+    case WT_FP16: launch_batch_f<WT_FP16>(dd, nt, ntile, parity, info->cpt, info->remap, f, s); break;
+    case WT_BF16: launch_batch_f<WT_BF16>(dd, nt, ntile, parity, info->cpt, info->remap, f, s); break;
+    case WT_FP64: launch_batch_f<WT_FP64>(dd, nt, ntile, parity, info->cpt, info->remap, f, s); break;
+  }
+  if (f) {
+    hipLaunchKernelGGL(k_reduce1, dim3((unsigned)info->max_chunks, (unsigned)nt), dim3(BLOCK), 0, s, rd);
+    hipLaunchKernelGGL(k_reduce2, dim3((unsigned)nt), dim3(BLOCK), 0, s, rd);
+  }
+  return check_launch("wt_batch_step");
+}
+
+}  // extern "C"

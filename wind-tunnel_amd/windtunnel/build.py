@@ -1,0 +1,88 @@
+"""Ahead-of-time build of libwt.so for gfx950 (hipcc, no JIT).
+
+Every translation unit is compiled with -ffp-contract=off (each floating-point
+operation rounds once, in source order) and with packed FP32 instructions
+disabled. The library links the HIP runtime by its SONAME, libamdhip64.so.7,
+so inside a process that imported torch first it binds to torch's own runtime.
+"""
+import os
+import shutil
+import subprocess
+import sys
+import tempfile
+
+PKG = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(PKG)                      # wind-tunnel_amd/
+REPO = os.path.dirname(ROOT)
+CSRC = os.path.join(ROOT, 'csrc')
+INCLUDE = os.path.join(REPO, 'include')
+LIB = os.path.join(PKG, 'libwt.so')
+ARCH = os.environ.get('WT_OFFLOAD_ARCH', 'gfx950')
+
+HIP_SOURCES = ['wt_step.hip', 'wt_aux.hip']
+CPP_SOURCES = ['wt_voxel.cpp']
+
+COMMON = ['-O3', '-std=c++17', '-fPIC', '-ffp-contract=off', '-Wall', '-Wno-unused-function',
+          '-I', INCLUDE, '-I', CSRC]
+HIP_FLAGS = ['--offload-arch=' + ARCH, '-Xclang', '-target-feature', '-Xclang', '-packed-fp32-ops',
+             ]
+
+
+def hipcc():
+    for c in (os.environ.get('HIPCC'), shutil.which('hipcc'), '/opt/rocm/bin/hipcc'):
+        if c and os.path.exists(c):
+            return c
+    raise RuntimeError('hipcc not found: cannot build libwt.so')
+
+
+def _run(cmd, verbose):
+    if verbose:
+        print(' '.join(cmd), flush=True)
+    r = subprocess.run(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True)
+    out = '\n'.join(l for l in r.stdout.splitlines() if 'packed-fp32-ops' not in l)
+    if r.returncode != 0:
+        raise RuntimeError(f'build step failed ({r.returncode}): {" ".join(cmd)}\n{out}')
+    if verbose and out.strip():
+        print(out, flush=True)
+
+
+def build(verbose=False, jobs=None):
+    """Compile csrc/ into windtunnel/libwt.so (always rebuilds)."""
+    cc = hipcc()
+    tmp = tempfile.mkdtemp(prefix='wt_build_')
+    try:
+        objs, cmds = [], []
+        for src in HIP_SOURCES:
+            o = os.path.join(tmp, src + '.o')
+            cmds.append([cc, *COMMON, *HIP_FLAGS, '-c', os.path.join(CSRC, src), '-o', o])
+            objs.append(o)
+        for src in CPP_SOURCES:
+            o = os.path.join(tmp, src + '.o')
+            cmds.append([cc, *COMMON, '-c', os.path.join(CSRC, src), '-o', o])
+            objs.append(o)
+        procs = []
+        for cmd in cmds:
+            if verbose:
+                print(' '.join(cmd), flush=True)
+            procs.append((cmd, subprocess.Popen(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT,
+                                                text=True)))
+        errors = []
+        for cmd, p in procs:
+            out, _ = p.communicate()
+            out = '\n'.join(l for l in out.splitlines() if 'packed-fp32-ops' not in l)
+            if p.returncode != 0:
+                errors.append(f'{" ".join(cmd)}\n{out}')
+            elif verbose and out.strip():
+                print(out, flush=True)
+        if errors:
+            raise RuntimeError('libwt build failed:\n' + '\n\n'.join(errors))
+        _run([cc, '-shared', '-fPIC', '--offload-arch=' + ARCH, *objs, '-o', LIB + '.tmp'], verbose)
+        os.replace(LIB + '.tmp', LIB)
+    finally:
+        shutil.rmtree(tmp, ignore_errors=True)
+    return LIB
+
+
+if __name__ == '__main__':
+    build(verbose=True)
+    sys.exit(0)

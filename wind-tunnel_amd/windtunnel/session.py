@@ -1,0 +1,101 @@
+"""A tunnel session: one resident tunnel with its particle rake, slice planes and
+force stream, driven step by step without host synchronization.
+
+This is the per-session unit of the serving loop and of the benchmark
+workloads. Per step it runs the fused step kernel (force fused on frame steps),
+advects the particles with dt = 1 (advect_kernel, lbm.py:233-263), and on frame
+steps extracts slice planes (slice_kernel, lbm.py:278-318) and queues
+asynchronous copies of slices and forces into pinned host buffers.
+"""
+import numpy as np
+import torch
+
+from ._lib import WT_FLAG_FORCES
+from .tunnel import Tunnel
+
+# slice field ranges of the viewer (web/js/main.js:15-20)
+FIELD_RANGES = {0: (0.0, 1.6), 1: (-1.2, 1.2), 2: (0.0, 12.0), 3: (-0.4, 1.4)}
+
+
+def inlet_rake(dims, ground=True):
+    """Inlet-wide particle rake: seedX = 2 and the y/z limits of
+    web/js/main.js:267-270 applied to the whole inlet plane."""
+    nx, ny, nz = dims
+    lo = 1.6 if ground else 1.5
+    y0, y1 = lo, ny - 2.5
+    z0, z1 = 1.5, nz - 2.5
+    return (2.0, (y0 + y1) / 2, (z0 + z1) / 2, (y1 - y0) / 2, (z1 - z0) / 2)
+
+
+class Session:
+    def __init__(self, dims, mask, form='fp32', uin=0.05, tau=0.51, smag=0.16, particles=0,
+                 slices=(), frame_every=10, lref=40.0, seed=0, device='cuda', ground=True, frontal=1):
+        self.dims = tuple(int(d) for d in dims)
+        self.uin, self.tau, self.smag = float(uin), float(tau), float(smag)
+        self.frame_every = int(frame_every)
+        self.lref = float(lref)
+        self.frontal = max(1, int(frontal))
+        self.T = Tunnel(dims, form, device)
+        self.T.init(uin)
+        self.T.set_solid(mask)
+        nx, ny, nz = self.dims
+        self.box = inlet_rake(dims, ground)
+        # maxAge = 2.2 nx / (U steps) with one step per advect (web/js/main.js:284)
+        self.max_age = 2.2 * nx / self.uin
+        self.num_p = int(particles)
+        self.seed = int(seed) * 1000003 + 1
+        self.step_count = 0
+        if self.num_p:
+            rng = np.random.default_rng(seed)
+            x, cy, cz, hy, hz = self.box
+            d = np.zeros((self.num_p, 4), np.float32)
+            d[:, 0] = x + rng.random(self.num_p) * 2.0
+            d[:, 1] = cy + (rng.random(self.num_p) * 2 - 1) * hy
+            d[:, 2] = cz + (rng.random(self.num_p) * 2 - 1) * hz
+            d[:, 3] = rng.random(self.num_p) * self.max_age * 0.5
+            self.pos = torch.from_numpy(d).to(device)
+            self.spd = torch.ones(self.num_p, dtype=torch.float32, device=device)
+        # slice planes: (axis, index, field)
+        self.slices = [tuple(s) for s in slices]
+        self.slice_dev, self.slice_host = [], []
+        for axis, index, field in self.slices:
+            w, h = {2: (nx, ny), 1: (nx, nz), 0: (nz, ny)}[axis]
+            self.slice_dev.append(torch.zeros(2 * w * h, dtype=torch.uint8, device=device))
+            self.slice_host.append(torch.zeros(2 * w * h, dtype=torch.uint8).pin_memory())
+        self.forces_host = torch.zeros(4, dtype=torch.float64).pin_memory()
+        self.force_log = []
+
+    @property
+    def n(self):
+        return self.T.n
+
+    def is_frame(self, k=None):
+        k = self.step_count if k is None else k
+        return self.frame_every > 0 and (k + 1) % self.frame_every == 0
+
+    def step(self, force=None, stream=None):
+        """One lattice step (+ particles); frame work on frame steps. Asynchronous."""
+        frame = self.is_frame()
+        want_force = frame if force is None else force
+        self.T.step(1, self.uin, self.tau, self.smag, flags=WT_FLAG_FORCES if want_force else 0, stream=stream)
+        if self.num_p:
+            self.seed += 1
+            self.T.advect(self.pos, self.spd, 1.0, self.uin, self.max_age, self.seed, self.box, stream=stream)
+        if frame:
+            self.emit_frame(stream)
+        elif want_force:
+            self.forces_host.copy_(self.T.sums, non_blocking=True)
+        self.step_count += 1
+
+    def emit_frame(self, stream=None):
+        for (axis, index, field), dev, host in zip(self.slices, self.slice_dev, self.slice_host):
+            vmin, vmax = FIELD_RANGES.get(field, (0.0, 1.0))
+            self.T.slice(dev, axis, index, field, self.uin, self.lref, vmin, vmax, stream=stream)
+            host.copy_(dev, non_blocking=True)
+        self.forces_host.copy_(self.T.sums, non_blocking=True)
+
+    def coefficients(self):
+        """Cd, Cl from the last copied forces (web/js/main.js:336-339); call after a sync."""
+        f = self.forces_host.numpy()
+        q = 0.5 * self.uin * self.uin * self.frontal
+        return f[0] / q, f[1] / q

@@ -1,0 +1,154 @@
+"""Device state of one wind tunnel and thin wrappers over the libwt kernels.
+
+torch provides the device buffers and the stream; every computation is a
+libwt call. Buffer layout (see include/wt.h): two SoA population buffers
+f[0], f[1] of 19*n elements of the storage form, a class byte per cell, the
+float4 macroscopic field mac (ux, uy, uz, rho), two face-layer snapshots, and
+the FP64 reduction slots.
+"""
+import ctypes
+
+import numpy as np
+import torch
+
+from . import _lib
+from ._lib import Grid, Params, SeedBox, check, stream_handle, FORMS, FORM_NAMES
+
+ELEM_DTYPE = {0: torch.float32, 1: torch.int16, 2: torch.int16, 3: torch.float64}
+# population weights w_i (server/lbm.py:18)
+W = np.array([1 / 3] + [1 / 18] * 6 + [1 / 36] * 12)
+
+
+def buffer_sizes(dims, form):
+    lib = _lib.load()
+    s = (ctypes.c_int64 * 6)()
+    check(lib.wt_buffer_sizes(int(dims[0]), int(dims[1]), int(dims[2]), int(form), s), 'wt_buffer_sizes')
+    return list(s)
+
+
+def state_bytes(dims, form='fp32'):
+    """Device bytes one tunnel occupies (admission footprint)."""
+    form = FORMS[form] if isinstance(form, str) else int(form)
+    lib = _lib.load()
+    return check(lib.wt_state_bytes(int(dims[0]), int(dims[1]), int(dims[2]), form), 'wt_state_bytes')
+
+
+def _ptr(t):
+    return ctypes.c_void_p(t.data_ptr())
+
+
+class Tunnel:
+    """One tunnel resident in HBM."""
+
+    def __init__(self, dims, form='fp32', device=None):
+        _lib.require_gpu()
+        self.lib = _lib.load()
+        self.form = FORMS[form] if isinstance(form, str) else int(form)
+        self.nx, self.ny, self.nz = (int(d) for d in dims)
+        self.n = self.nx * self.ny * self.nz
+        self.device = torch.device(device if device is not None else 'cuda')
+        sz = buffer_sizes(dims, self.form)
+        dev = self.device
+        self.f = [torch.empty(sz[0], dtype=torch.uint8, device=dev) for _ in range(2)]
+        self.cls = torch.zeros(sz[1], dtype=torch.uint8, device=dev)
+        self.mac = torch.zeros((self.n, 4), dtype=torch.float32, device=dev)
+        self.shell = [torch.empty(sz[3], dtype=torch.uint8, device=dev) for _ in range(2)]
+        self.partials = torch.zeros(sz[4] // 8, dtype=torch.float64, device=dev)
+        self.sums = torch.zeros(4, dtype=torch.float64, device=dev)
+        self.grid = Grid(self.nx, self.ny, self.nz, self.form,
+                         (ctypes.c_void_p * 2)(self.f[0].data_ptr(), self.f[1].data_ptr()),
+                         self.cls.data_ptr(), self.mac.data_ptr(),
+                         (ctypes.c_void_p * 2)(self.shell[0].data_ptr(), self.shell[1].data_ptr()),
+                         self.partials.data_ptr(), self.sums.data_ptr())
+        self.gp = ctypes.byref(self.grid)
+        self.cur = 0
+
+    @property
+    def form_name(self):
+        return FORM_NAMES[self.form]
+
+    @property
+    def nbytes(self):
+        return sum(t.numel() * t.element_size() for t in
+                   (*self.f, self.cls, self.mac, *self.shell, self.partials, self.sums))
+
+    # ---- state -------------------------------------------------------------
+    def init(self, uin, stream=None):
+        check(self.lib.wt_init(self.gp, float(uin), stream_handle(stream)), 'wt_init')
+        self.cur = 0
+
+    def set_solid(self, mask, stream=None):
+        mask = np.ascontiguousarray(np.asarray(mask).reshape(-1), dtype=np.uint8)
+        if mask.size != self.n:
+            raise ValueError(f'solid mask has {mask.size} cells, grid has {self.n}')
+        dev = torch.from_numpy(mask).to(self.device, non_blocking=False)
+        check(self.lib.wt_set_solid(self.gp, _ptr(dev), stream_handle(stream)), 'wt_set_solid')
+        torch.cuda.current_stream().synchronize() if stream is None else stream.synchronize()
+
+    # ---- hot path ----------------------------------------------------------
+    def step(self, nsteps, uin, tau, smag, flags=0, stream=None):
+        p = Params(float(uin), float(tau), float(smag))
+        check(self.lib.wt_step(self.gp, self.cur, int(nsteps), ctypes.byref(p), int(flags),
+                               stream_handle(stream)), 'wt_step')
+        self.cur ^= int(nsteps) & 1
+
+    def forces(self, stream=None):
+        """Momentum-exchange force on f[cur] into self.sums[0:3] (device)."""
+        check(self.lib.wt_forces(self.gp, self.cur, stream_handle(stream)), 'wt_forces')
+        return self.sums
+
+    def link_classes(self, stream=None):
+        out = torch.empty(self.n, dtype=torch.int64, device=self.device)
+        check(self.lib.wt_link_classes(self.gp, _ptr(out), stream_handle(stream)), 'wt_link_classes')
+        return out
+
+    # ---- views ---------------------------------------------------------------
+    def raw_populations(self, which=None):
+        """f[which] (default f[cur]) as its storage dtype, shape (19, n), on the device."""
+        k = self.cur if which is None else which
+        return self.f[k].view(ELEM_DTYPE[self.form]).view(19, self.n)
+
+    def populations(self, which=None, dtype=np.float64):
+        """f_i of every cell on the host, shape (19, n), reconstructed from the storage form."""
+        raw = self.raw_populations(which).cpu().numpy()
+        if self.form == 0:
+            return raw.astype(dtype)
+        if self.form == 3:
+            return raw.astype(dtype)
+        u16 = raw.view(np.uint16)
+        if self.form == 1:
+            g = u16.view(np.float16).astype(np.float32)
+        else:
+            g = (u16.astype(np.uint32) << 16).view(np.float32)
+        return (W.astype(np.float32)[:, None] + g).astype(dtype)
+
+    # ---- particles, slices, surface -------------------------------------------
+    def advect(self, pos, spd, dt, uin, max_age, seed, box, stream=None):
+        sb = SeedBox(*(float(v) for v in box))
+        check(self.lib.wt_advect(self.gp, _ptr(pos), _ptr(spd), int(pos.shape[0]), float(dt), float(uin),
+                                 float(max_age), int(seed) & 0xffffffff, ctypes.byref(sb),
+                                 stream_handle(stream)), 'wt_advect')
+
+    def pack_particles(self, pos, spd, q16, q8, stream=None):
+        check(self.lib.wt_pack_particles(self.gp, _ptr(pos), _ptr(spd), _ptr(q16), _ptr(q8),
+                                         int(pos.shape[0]), stream_handle(stream)), 'wt_pack_particles')
+
+    def slice(self, out, axis, index, field, uin, lref, vmin, vmax, stream=None):
+        check(self.lib.wt_slice(self.gp, _ptr(out), int(axis), int(index), int(field), float(uin),
+                                float(lref), float(vmin), float(vmax), stream_handle(stream)), 'wt_slice')
+
+    def cp(self, verts, norms, out, uin, cp_range, stream=None):
+        check(self.lib.wt_cp(self.gp, _ptr(verts), _ptr(norms), _ptr(out), int(verts.shape[0]), float(uin),
+                             float(cp_range), stream_handle(stream)), 'wt_cp')
+
+
+def set_launch_shape(form, cpt, remap):
+    form = FORMS[form] if isinstance(form, str) else int(form)
+    check(_lib.load().wt_set_launch_shape(form, int(cpt), int(remap)), 'wt_set_launch_shape')
+
+
+def get_launch_shape(form):
+    form = FORMS[form] if isinstance(form, str) else int(form)
+    c, r = ctypes.c_int32(), ctypes.c_int32()
+    check(_lib.load().wt_get_launch_shape(form, ctypes.byref(c), ctypes.byref(r)), 'wt_get_launch_shape')
+    return c.value, r.value

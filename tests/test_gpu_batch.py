@@ -1,0 +1,88 @@
+"""Batched tunnels: every tunnel advanced inside one batched launch must equal
+the same tunnel advanced alone, bit for bit (populations, macroscopic field and
+the FP64 force/mass sums), also when replayed from a captured graph."""
+import numpy as np
+import pytest
+import torch
+
+pytestmark = pytest.mark.gpu
+
+SIZES = [(32, 16, 16), (48, 24, 24), (40, 20, 22), (56, 28, 24), (64, 32, 32)]
+PARAMS = [(0.05, 0.52, 0.16), (0.08, 0.60, 0.17), (0.03, 0.51, 0.16), (0.06, 0.55, 0.10), (0.04, 0.53, 0.2)]
+
+
+def _mask(dims, seed):
+    from windtunnel import geometry
+    nx, ny, nz = dims
+    rng = np.random.default_rng(seed)
+    if seed % 2:
+        c = (nx * 0.3, ny * 0.5, nz * 0.5)
+        return geometry.sphere(dims, c, min(ny, nz) * 0.4)[0]
+    ells = [[nx * 0.35, ny * rng.uniform(0.3, 0.6), nz * 0.5, nx * 0.12, ny * 0.2, nz * 0.2]]
+    return geometry.ellipsoids(ells, dims, ground=bool(seed % 4 == 0))[0]
+
+
+def _make(form, pre_steps=()):
+    from windtunnel.tunnel import Tunnel
+    ts = []
+    for k, (dims, p) in enumerate(zip(SIZES, PARAMS)):
+        t = Tunnel(dims, form)
+        t.init(p[0])
+        t.set_solid(_mask(dims, k))
+        if k in pre_steps:
+            t.step(1, *p)
+        ts.append(t)
+    return ts
+
+
+def _state(t):
+    return (t.raw_populations().cpu().numpy().copy(), t.mac.cpu().numpy().copy(), t.sums.cpu().numpy().copy())
+
+
+def _same(a, b):
+    return all(np.array_equal(np.ascontiguousarray(x).view(np.uint8), np.ascontiguousarray(y).view(np.uint8))
+               for x, y in zip(a, b))
+
+
+@pytest.mark.parametrize('form', ['fp32', 'fp16'])
+def test_batch_equals_alone(gpu, form):
+    from windtunnel._lib import WT_FLAG_FORCES, WT_FLAG_MASS
+    from windtunnel.batch import Batch
+    alone = _make(form, pre_steps=(2,))
+    mixed = _make(form, pre_steps=(2,))
+    b = Batch(mixed, PARAMS)
+    flags = WT_FLAG_FORCES | WT_FLAG_MASS
+    for k in range(9):
+        b.step(flags)
+        for t, p in zip(alone, PARAMS):
+            t.step(1, *p, flags=flags)
+        torch.cuda.synchronize()
+        for ta, tm in zip(alone, mixed):
+            assert ta.cur == tm.cur
+            assert _same(_state(ta), _state(tm)), (form, k, ta.nx)
+
+
+def test_batch_graph_replay_equals_eager(gpu):
+    from windtunnel._lib import WT_FLAG_FORCES
+    from windtunnel.batch import Batch
+    eager = _make('fp32')
+    graphed = _make('fp32')
+    be = Batch(eager, PARAMS)
+    bg = Batch(graphed, PARAMS)
+    bg.capture(WT_FLAG_FORCES)
+    for rep in range(5):
+        be.step(WT_FLAG_FORCES)
+        be.step(WT_FLAG_FORCES)
+        bg.replay()
+        torch.cuda.synchronize()
+        # eager host work between replays
+        _ = torch.arange(1000, device='cuda').sum().item()
+        for ta, tb in zip(eager, graphed):
+            assert _same(_state(ta), _state(tb)), rep
+
+
+def test_batch_rejects_mixed_forms(gpu):
+    from windtunnel.batch import Batch
+    from windtunnel.tunnel import Tunnel
+    with pytest.raises(ValueError):
+        Batch([Tunnel((8, 8, 8), 'fp32'), Tunnel((8, 8, 8), 'fp16')], [(0.05, 0.52, 0.16)] * 2)

@@ -1,0 +1,81 @@
+"""Many resident tunnels advanced together: one launch of the batched step
+kernel per lattice step for all tunnels of one storage form.
+
+The descriptor table (per-tunnel buffers, sizes, parameters, tile offsets) is
+uploaded once per batch composition (wt_batch_prepare); each step is then a
+kernel launch only (wt_batch_step), so a step of the whole batch can be
+captured in a HIP graph. Graph replays advance two steps (parity 0 then 1),
+returning every tunnel to the same buffer parity.
+"""
+import ctypes
+
+import torch
+
+from . import _lib
+from ._lib import BatchInfo, Grid, Params, check, stream_handle
+
+
+class Batch:
+    def __init__(self, tunnels, params):
+        """tunnels: list of Tunnel (same storage form); params: list of (uin, tau, smag)."""
+        if not tunnels:
+            raise ValueError('empty batch')
+        forms = {t.form for t in tunnels}
+        if len(forms) != 1:
+            raise ValueError(f'a batch holds one storage form, got {sorted(forms)}')
+        self.lib = _lib.load()
+        self.tunnels = list(tunnels)
+        self.params = [tuple(float(v) for v in p) for p in params]
+        nt = len(self.tunnels)
+        nbytes = check(self.lib.wt_batch_workspace_bytes(nt), 'wt_batch_workspace_bytes')
+        self.workspace = torch.empty(nbytes, dtype=torch.uint8, device=self.tunnels[0].device)
+        self.info = BatchInfo()
+        self.parity = 0
+        self.graph = None
+        self.prepare()
+
+    def prepare(self, stream=None):
+        nt = len(self.tunnels)
+        grids = (Grid * nt)(*[t.grid for t in self.tunnels])
+        prm = (Params * nt)(*[Params(*p) for p in self.params])
+        cur = (ctypes.c_int32 * nt)(*[t.cur for t in self.tunnels])
+        check(self.lib.wt_batch_prepare(grids, prm, cur, nt, ctypes.c_void_p(self.workspace.data_ptr()),
+                                        ctypes.byref(self.info), stream_handle(stream)), 'wt_batch_prepare')
+        self.parity = 0
+        self.graph = None
+
+    @property
+    def cells(self):
+        return sum(t.n for t in self.tunnels)
+
+    def step(self, flags=0, stream=None):
+        """One lattice step of every tunnel (asynchronous)."""
+        check(self.lib.wt_batch_step(ctypes.c_void_p(self.workspace.data_ptr()), ctypes.byref(self.info),
+                                     self.parity, int(flags), stream_handle(stream)), 'wt_batch_step')
+        self.parity ^= 1
+        for t in self.tunnels:
+            t.cur ^= 1
+
+    def capture(self, flags=0):
+        """Capture two batch steps (parity 0 and 1) in a HIP graph."""
+        if self.parity != 0:
+            raise RuntimeError('capture at even parity')
+        s = torch.cuda.Stream()
+        s.wait_stream(torch.cuda.current_stream())
+        g = torch.cuda.CUDAGraph()
+        with torch.cuda.stream(s):
+            with torch.cuda.graph(g, stream=s):
+                for p in (0, 1):
+                    check(self.lib.wt_batch_step(ctypes.c_void_p(self.workspace.data_ptr()),
+                                                 ctypes.byref(self.info), p, int(flags), stream_handle(s)),
+                          'wt_batch_step')
+        torch.cuda.current_stream().wait_stream(s)
+        self.graph = g
+        return g
+
+    def replay(self):
+        """Two steps of every tunnel from the captured graph."""
+        if self.graph is None:
+            raise RuntimeError('no captured graph')
+        self.graph.replay()
+        # two flips leave every tunnel's cur unchanged

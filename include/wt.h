@@ -157,6 +157,13 @@ int wt_cp(const wt_grid* g, const float* verts, const float* norms, int8_t* out,
  * force (i = 1..18). Mirrors lbm.py:81-84, 114-124, 185-188. */
 int wt_link_classes(const wt_grid* g, uint64_t* out_dev, void* stream);
 
+/* ---- measurement ---------------------------------------------------------- */
+/* Streaming device-to-device copy of `bytes` (multiple of 16, 16-byte aligned
+ * buffers) with `blocks` workgroups (<= 0: a default filling the chip). A plain
+ * copy of known size: the reference point for achievable HBM bandwidth and for
+ * scaling the FETCH_SIZE / WRITE_SIZE counters. No counterpart in the reference. */
+int wt_stream_copy(void* dst, const void* src, int64_t bytes, int blocks, void* stream);
+
 /* ---- geometry (host code) ------------------------------------------------- */
 /* Voxelize a triangle soup given in lattice coordinates (9 floats per
  * triangle) into out_host[n] exactly as voxelize.js:5-115 does: ray parity

@@ -12,7 +12,8 @@ import threading
 import torch  # noqa: F401  (must precede loading libwt, see above)
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(HERE, 'libwt.so')
+# WT_LIB_PATH selects another build of the same library (e.g. a tuning variant)
+LIB_PATH = os.environ.get('WT_LIB_PATH') or os.path.join(HERE, 'libwt.so')
 
 WT_FP32, WT_FP16, WT_BF16, WT_FP64 = 0, 1, 2, 3
 FORMS = {'fp32': WT_FP32, 'fp16': WT_FP16, 'bf16': WT_BF16, 'fp64': WT_FP64}
@@ -69,6 +70,7 @@ SIGNATURES = {
     'wt_slice': (c_i32, [P(Grid), c_vp, c_i32, c_i32, c_i32, c_f32, c_f32, c_f32, c_f32, c_vp]),
     'wt_cp': (c_i32, [P(Grid), c_vp, c_vp, c_vp, c_i64, c_f32, c_f32, c_vp]),
     'wt_link_classes': (c_i32, [P(Grid), c_vp, c_vp]),
+    'wt_stream_copy': (c_i32, [c_vp, c_vp, c_i64, c_i32, c_vp]),
     'wt_voxelize': (c_i32, [c_vp, c_i64, c_i32, c_i32, c_i32, c_i32, c_vp, P(c_i64)]),
     'wt_ellipsoids': (c_i32, [c_vp, c_i32, c_i32, c_i32, c_i32, c_i32, c_vp, P(c_i64)]),
 }

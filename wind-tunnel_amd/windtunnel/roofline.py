@@ -1,0 +1,104 @@
+"""Kernel-level measurements on the GPU: achievable HBM bandwidth of a plain
+copy, the fused step kernel's time per launch for each storage form, and a
+sweep of its launch shapes.
+
+    python -m windtunnel.roofline [--forms fp32,fp16] [--steps 100] [--sweep] [--copy-gb 4]
+
+Prints one JSON object per measurement. Algorithmic bytes per cell of the step:
+19 populations read + 19 written, 1 class byte, 16 B macroscopic store
+(FP32 169, FP16/BF16 93, FP64 321).
+"""
+import argparse
+import ctypes
+import json
+import sys
+
+import numpy as np
+import torch
+
+from . import _lib, geometry
+from .tunnel import Tunnel, get_launch_shape, set_launch_shape
+
+BYTES_PER_CELL = {'fp32': 169, 'fp16': 93, 'bf16': 93, 'fp64': 321}
+PEAK_GBS = 8000.0
+
+
+def copy_bandwidth(gbytes=4.0, reps=20, blocks=0):
+    lib = _lib.load()
+    nbytes = int(gbytes * 1e9) // 16 * 16
+    a = torch.empty(nbytes, dtype=torch.uint8, device='cuda')
+    b = torch.empty(nbytes, dtype=torch.uint8, device='cuda')
+    a.fill_(1)
+    s = _lib.stream_handle()
+    for _ in range(3):
+        _lib.check(lib.wt_stream_copy(ctypes.c_void_p(b.data_ptr()), ctypes.c_void_p(a.data_ptr()), nbytes,
+                                      blocks, s), 'wt_stream_copy')
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(reps):
+        lib.wt_stream_copy(ctypes.c_void_p(b.data_ptr()), ctypes.c_void_p(a.data_ptr()), nbytes, blocks, s)
+    e1.record()
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1) / reps
+    del a, b
+    torch.cuda.empty_cache()
+    return {'what': 'stream_copy', 'bytes_moved': 2 * nbytes, 'ms': ms, 'gbs': 2 * nbytes / (ms * 1e-3) / 1e9}
+
+
+def c2_mask(dims=(512, 256, 256), seed=2025):
+    ells = geometry.car_body(dims, seed)
+    return geometry.ellipsoids(ells, dims, ground=True)[0]
+
+
+def step_timing(form, dims, mask, steps=100, warmup=10, flags=0, tunnel=None):
+    t = tunnel or Tunnel(dims, form)
+    if tunnel is None:
+        t.init(0.05)
+        t.set_solid(mask)
+    for _ in range(warmup):
+        t.step(1, 0.05, 0.51, 0.16, flags)
+    torch.cuda.synchronize()
+    evs = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in range(steps)]
+    for k in range(steps):
+        evs[k][0].record()
+        t.step(1, 0.05, 0.51, 0.16, flags)
+        evs[k][1].record()
+    torch.cuda.synchronize()
+    ms = np.array([a.elapsed_time(b) for a, b in evs])
+    n = t.n
+    med = float(np.median(ms))
+    return t, {'what': 'step', 'form': form, 'dims': list(dims), 'cpt_remap': list(get_launch_shape(form)),
+               'flags': flags, 'ms_mean': float(ms.mean()), 'ms_median': med, 'ms_min': float(ms.min()),
+               'mlups': n / (med * 1e-3) / 1e6,
+               'gbs': BYTES_PER_CELL[form] * n / (med * 1e-3) / 1e9,
+               'frac': BYTES_PER_CELL[form] * n / (med * 1e-3) / 1e9 / PEAK_GBS}
+
+
+def main(argv=None):
+    ap = argparse.ArgumentParser()
+    ap.add_argument('--forms', default='fp32,fp16')
+    ap.add_argument('--dims', default='512,256,256')
+    ap.add_argument('--steps', type=int, default=100)
+    ap.add_argument('--sweep', action='store_true')
+    ap.add_argument('--copy-gb', type=float, default=4.0)
+    args = ap.parse_args(argv)
+    dims = tuple(int(v) for v in args.dims.split(','))
+    if args.copy_gb > 0:
+        print(json.dumps(copy_bandwidth(args.copy_gb)), flush=True)
+    mask = c2_mask(dims)
+    for form in args.forms.split(','):
+        shapes = [(1, 0), (1, 1), (2, 0), (2, 1), (4, 0), (4, 1)] if args.sweep else [get_launch_shape(form)]
+        t = None
+        old = get_launch_shape(form)
+        for cpt, remap in shapes:
+            set_launch_shape(form, cpt, remap)
+            t, r = step_timing(form, dims, mask, args.steps, tunnel=t)
+            print(json.dumps(r), flush=True)
+        set_launch_shape(form, *old)
+        del t
+        torch.cuda.empty_cache()
+    return 0
+
+
+if __name__ == '__main__':
+    sys.exit(main())

@@ -132,6 +132,8 @@ def main():
     ap.add_argument('--warmup', type=int, default=20)
     ap.add_argument('--forms', default='fp32,fp16')
     ap.add_argument('--no-cpu', action='store_true')
+    ap.add_argument('--c4-steps', type=int, default=500)
+    ap.add_argument('--serve-seconds', type=float, default=30.0)
     args = ap.parse_args()
     if args.gpus != 1:
         print(f'bench.py measures one GPU: this build has no multi-GPU path (asked for --gpus {args.gpus}). '
@@ -151,6 +153,21 @@ def main():
         log(f'GPU run {form}: warmup {args.warmup}, {args.steps} steps ...')
         results[form] = run_gpu(form, args.steps, args.warmup, mask, counts, lref)
         log(json.dumps(results[form]))
+    c4 = None
+    if args.c4_steps > 0:
+        from windtunnel.workloads import run_config4
+        log(f'config-4 batch: 48 tunnels, {args.c4_steps} steps ...')
+        c4 = run_config4(steps=args.c4_steps, warmup=20, graphs=True)
+        log(json.dumps(c4))
+        torch.cuda.empty_cache()
+    serving = None
+    if args.serve_seconds > 0:
+        from windtunnel.serve_bench import run as serve_run
+        log(f'serving mix: Poisson 2/s, {args.serve_seconds:.0f} s ...')
+        serving = serve_run(duration=args.serve_seconds, rate=2.0, seed=0, graphs=False,
+                            time_limit=args.serve_seconds, progress=False)
+        log(json.dumps(serving))
+        torch.cuda.empty_cache()
     cpu = None
     if not args.no_cpu:
         log('CPU baseline ...')
@@ -183,6 +200,20 @@ def main():
                       'achieved_gbs': round(rr['achieved_gbs'], 1), 'bytes_per_cell': BYTES_PER_CELL[form],
                       'traffic': load_traffic(form)}
     line['cd_cl'] = {f: [round(v['cd'], 4), round(v['cl'], 4)] for f, v in results.items()}
+    if c4:
+        line['config4_batch'] = {'value': round(c4['mlups'], 1), 'unit': 'MLUPS', 'tunnels': c4['tunnels'],
+                                 'cells': c4['cells'], 'steps': c4['steps'], 'ms_per_step': round(c4['ms_per_step'], 4),
+                                 'launch': 'one batched launch per step (graph replay of 2 steps), forces every step'}
+    if serving:
+        line['serving'] = {'step_latency_ms_p50': round(serving['step_latency_ms']['p50'], 3),
+                           'step_latency_ms_p99': round(serving['step_latency_ms']['p99'], 3),
+                           'queue_delay_s_p50': round(serving['queue_delay_s']['p50'], 3),
+                           'queue_delay_s_p99': round(serving['queue_delay_s']['p99'], 3),
+                           'mlups': round(serving['mlups'], 1), 'window_s': args.serve_seconds,
+                           'sessions_admitted': len(serving.get('admitted', [])) or None,
+                           'sessions_done': serving['sessions_done'],
+                           'workload': 'Poisson 2 sessions/s, 0.5M-33M cells log-uniform, 200-5000 steps, '
+                                       'half fp32 half fp16, forces every step, slice every 20 steps'}
     print(json.dumps(line), flush=True)
 
 

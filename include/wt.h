@@ -120,9 +120,12 @@ typedef struct wt_batch_info {
 /* Device workspace bytes for a batch of ntunnels (descriptor tables). */
 int64_t wt_batch_workspace_bytes(int ntunnels);
 /* Upload the descriptor table of a batch: grids[k] with params[k], current
- * buffer cur[k]. All tunnels share one storage form. Synchronizes `stream`. */
+ * buffer cur[k]. All tunnels share one storage form. If sums_dev is not NULL
+ * (device array of 2*ntunnels*4 doubles) the reduced sums of tunnel k after a
+ * step of parity p land at sums_dev[(p*ntunnels + k)*4 ...], so one copy
+ * fetches the whole batch; otherwise in grids[k].sums. Synchronizes `stream`. */
 int wt_batch_prepare(const wt_grid* grids, const wt_params* params, const int* cur, int ntunnels,
-                     void* workspace, wt_batch_info* info, void* stream);
+                     double* sums_dev, void* workspace, wt_batch_info* info, void* stream);
 /* One step of every tunnel of the prepared batch; with parity p tunnel k
  * reads buffer cur[k]^p. flags as for wt_step; sums land in each grid's sums.
  * Kernel launches only: a caller may capture it in a graph. */
@@ -159,7 +162,7 @@ int wt_link_classes(const wt_grid* g, uint64_t* out_dev, void* stream);
 
 /* ---- measurement ---------------------------------------------------------- */
 /* Streaming device-to-device copy of `bytes` (multiple of 16, 16-byte aligned
- * buffers) with `blocks` workgroups (<= 0: a default filling the chip). A plain
+ * buffers), one 16-byte element per thread; `blocks` is ignored. A plain
  * copy of known size: the reference point for achievable HBM bandwidth and for
  * scaling the FETCH_SIZE / WRITE_SIZE counters. No counterpart in the reference. */
 int wt_stream_copy(void* dst, const void* src, int64_t bytes, int blocks, void* stream);

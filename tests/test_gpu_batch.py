@@ -57,9 +57,11 @@ def test_batch_equals_alone(gpu, form):
         for t, p in zip(alone, PARAMS):
             t.step(1, *p, flags=flags)
         torch.cuda.synchronize()
-        for ta, tm in zip(alone, mixed):
+        bs = b.last_sums().cpu().numpy()
+        for j, (ta, tm) in enumerate(zip(alone, mixed)):
             assert ta.cur == tm.cur
-            assert _same(_state(ta), _state(tm)), (form, k, ta.nx)
+            assert _same(_state(ta)[:2], _state(tm)[:2]), (form, k, ta.nx)
+            assert _same([ta.sums.cpu().numpy()], [bs[j]]), (form, k, ta.nx)
 
 
 def test_batch_graph_replay_equals_eager(gpu):
@@ -72,13 +74,15 @@ def test_batch_graph_replay_equals_eager(gpu):
     bg.capture(WT_FLAG_FORCES)
     for rep in range(5):
         be.step(WT_FLAG_FORCES)
+        s0 = be.last_sums().clone()
         be.step(WT_FLAG_FORCES)
         bg.replay()
         torch.cuda.synchronize()
         # eager host work between replays
         _ = torch.arange(1000, device='cuda').sum().item()
         for ta, tb in zip(eager, graphed):
-            assert _same(_state(ta), _state(tb)), rep
+            assert _same(_state(ta)[:2], _state(tb)[:2]), rep
+        assert _same([s0.cpu().numpy(), be.last_sums().cpu().numpy()], [bg.sums[0].cpu().numpy(), bg.sums[1].cpu().numpy()])
 
 
 def test_batch_rejects_mixed_forms(gpu):

@@ -136,16 +136,13 @@ def test_launch_shape_does_not_change_results(gpu, cpt, remap):
     from windtunnel._lib import WT_FLAG_FORCES, WT_FLAG_MASS
     dims = (70, 26, 22)    # n not a multiple of 256 or 1024
     mask = _car_mask(dims, seed=6)
-    old = T.get_launch_shape('fp32')
     res = []
-    try:
-        for shape in ((1, 1), (cpt, remap)):
-            T.set_launch_shape('fp32', *shape)
-            t = _tunnel(dims, 'fp32', mask, 0.05)
-            t.step(11, 0.05, 0.52, 0.17, flags=WT_FLAG_FORCES | WT_FLAG_MASS)
-            res.append((t.raw_populations().cpu().numpy(), t.mac.cpu().numpy(), t.sums.cpu().numpy()))
-    finally:
-        T.set_launch_shape('fp32', *old)
+    for shape in ((1, 1), (cpt, remap)):
+        t = _tunnel(dims, 'fp32', mask, 0.05)
+        t.shape = shape
+        t.step(11, 0.05, 0.52, 0.17, flags=WT_FLAG_FORCES | WT_FLAG_MASS)
+        assert T.get_launch_shape('fp32') == shape
+        res.append((t.raw_populations().cpu().numpy(), t.mac.cpu().numpy(), t.sums.cpu().numpy()))
     for a, b in zip(res[0], res[1]):
         assert _raw_equal(a, b)
 

@@ -517,7 +517,8 @@ __global__ __launch_bounds__(BLOCK) void k_step_batch(const StepDesc* __restrict
 // ---- reductions of the slots ---------------------------------------------------------
 struct RedDesc {
   double* partials;  // nslots slots, then nchunks first-level sums
-  double* sums;      // 4 doubles
+  double* sums;      // 4 doubles (step parity 0)
+  double* sums_odd;  // 4 doubles (step parity 1)
   uint32_t nslots;
 };
 
@@ -553,7 +554,7 @@ __global__ __launch_bounds__(BLOCK) void k_reduce1(const RedDesc* __restrict__ r
 }
 
 // Second level: block t sums its first-level chunks in order.
-__global__ __launch_bounds__(BLOCK) void k_reduce2(const RedDesc* __restrict__ rd) {
+__global__ __launch_bounds__(BLOCK) void k_reduce2(const RedDesc* __restrict__ rd, int parity) {
   __shared__ double lds[4];
   const RedDesc r = rd[blockIdx.x];
   const uint32_t nchunks = (r.nslots + CHUNK - 1) / CHUNK;
@@ -565,7 +566,7 @@ __global__ __launch_bounds__(BLOCK) void k_reduce2(const RedDesc* __restrict__ r
 #pragma unroll
   for (int c = 0; c < 4; c++) {
     const double t = block_sum(v[c], lds);
-    if (threadIdx.x == 0) r.sums[c] = t;
+    if (threadIdx.x == 0) (parity ? r.sums_odd : r.sums)[c] = t;
   }
 }
 
@@ -763,7 +764,7 @@ int check_grid(const wt_grid* g, bool need_state) {
 static int reduce_one(const wt_grid* g, hipStream_t s) {
   const int64_t n = (int64_t)g->nx * g->ny * g->nz;
   if (!g->partials || !g->sums) return fail(-1, "grid has no reduction buffers");
-  RedDesc r{g->partials, g->sums, (uint32_t)nslots_of(n)};
+  RedDesc r{g->partials, g->sums, g->sums, (uint32_t)nslots_of(n)};
   hipLaunchKernelGGL(k_reduce1_one, dim3((unsigned)nchunks_of(n)), dim3(BLOCK), 0, s, r);
   hipLaunchKernelGGL(k_reduce2_one, dim3(1), dim3(BLOCK), 0, s, r);
   return check_launch("reduce");
@@ -963,7 +964,7 @@ int64_t wt_batch_workspace_bytes(int ntunnels) {
 }
 
 int wt_batch_prepare(const wt_grid* grids, const wt_params* params, const int* cur, int ntunnels,
-                     void* workspace, wt_batch_info* info, void* stream) {
+                     double* sums_dev, void* workspace, wt_batch_info* info, void* stream) {
   if (!grids || !params || !cur || !workspace || !info || ntunnels <= 0)
     return fail(-1, "null argument or empty batch");
   const int form = grids[0].form;
@@ -985,7 +986,11 @@ int wt_batch_prepare(const wt_grid* grids, const wt_params* params, const int* c
     tile += d.ntiles;
     if (tile >= (1ull << 31)) return fail(-1, "batch has too many tiles");
     ds[k] = d;
-    rs[k] = RedDesc{grids[k].partials, grids[k].sums, (uint32_t)nslots_of(d.n)};
+    if (sums_dev)
+      rs[k] = RedDesc{grids[k].partials, sums_dev + 4 * (size_t)k, sums_dev + 4 * ((size_t)ntunnels + k),
+                      (uint32_t)nslots_of(d.n)};
+    else
+      rs[k] = RedDesc{grids[k].partials, grids[k].sums, grids[k].sums, (uint32_t)nslots_of(d.n)};
     max_chunks = std::max<int64_t>(max_chunks, nchunks_of(d.n));
   }
   hipStream_t s = (hipStream_t)stream;
@@ -1026,7 +1031,7 @@ int wt_batch_step(void* workspace, const wt_batch_info* info, int parity, int fl
   }
   if (f) {
     hipLaunchKernelGGL(k_reduce1, dim3((unsigned)info->max_chunks, (unsigned)nt), dim3(BLOCK), 0, s, rd);
-    hipLaunchKernelGGL(k_reduce2, dim3((unsigned)nt), dim3(BLOCK), 0, s, rd);
+    hipLaunchKernelGGL(k_reduce2, dim3((unsigned)nt), dim3(BLOCK), 0, s, rd, parity & 1);
   }
   return check_launch("wt_batch_step");
 }

@@ -61,7 +61,7 @@ SIGNATURES = {
     'wt_forces': (c_i32, [P(Grid), c_i32, c_vp]),
     'wt_read_sums': (c_i32, [P(Grid), P(ctypes.c_double), c_vp]),
     'wt_batch_workspace_bytes': (c_i64, [c_i32]),
-    'wt_batch_prepare': (c_i32, [P(Grid), P(Params), P(c_i32), c_i32, c_vp, P(BatchInfo), c_vp]),
+    'wt_batch_prepare': (c_i32, [P(Grid), P(Params), P(c_i32), c_i32, c_vp, c_vp, P(BatchInfo), c_vp]),
     'wt_batch_step': (c_i32, [c_vp, P(BatchInfo), c_i32, c_i32, c_vp]),
     'wt_set_launch_shape': (c_i32, [c_i32, c_i32, c_i32]),
     'wt_get_launch_shape': (c_i32, [c_i32, P(c_i32), P(c_i32)]),

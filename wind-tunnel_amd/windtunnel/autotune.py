@@ -1,0 +1,106 @@
+"""Launch-shape autotuner for the fused step kernel.
+
+Shapes: cells per thread (1, 2 or 4 consecutive 256-cell slots per
+workgroup) x tile order (plain or XCD-contiguous). Results do not depend on the
+shape (the force/mass slots are reduced identically), only the speed does.
+For each storage form and size class the tuner times every shape on a
+representative tunnel and stores the fastest in tuning.json beside this file;
+tunnels look their shape up by size class at launch time.
+
+    python -m windtunnel.autotune [--forms fp32,fp16,bf16,fp64] [--steps 40]
+"""
+import argparse
+import json
+import os
+import sys
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+TUNING_PATH = os.environ.get('WT_TUNING_PATH') or os.path.join(HERE, 'tuning.json')
+
+# size classes by cell count, with the tunnel each is tuned on
+CLASSES = [
+    ('S', 0, 200_000, (64, 32, 32)),
+    ('M', 200_000, 2_000_000, (128, 64, 64)),
+    ('L', 2_000_000, 12_000_000, (256, 128, 128)),
+    ('XL', 12_000_000, 1 << 31, (512, 256, 256)),
+]
+SHAPES = [(1, 0), (1, 1), (2, 0), (2, 1), (4, 0), (4, 1)]
+DEFAULT = (1, 0)
+
+_table = None
+
+
+def size_class(cells):
+    for name, lo, hi, _ in CLASSES:
+        if lo <= cells < hi:
+            return name
+    return CLASSES[-1][0]
+
+
+def load():
+    global _table
+    if _table is None:
+        try:
+            with open(TUNING_PATH) as fh:
+                _table = json.load(fh).get('shapes', {})
+        except (OSError, ValueError):
+            _table = {}
+    return _table
+
+
+def shape_for(form_name, cells):
+    t = load().get(form_name, {})
+    v = t.get(size_class(cells))
+    return tuple(v) if v else DEFAULT
+
+
+def tune(forms, steps=40):
+    import numpy as np
+    import torch
+    from . import geometry
+    from .roofline import step_timing
+    from .tunnel import Tunnel, get_launch_shape, set_launch_shape
+    out = {'arch': torch.cuda.get_device_properties(0).gcnArchName, 'shapes': {}, 'timings': {}}
+    for form in forms:
+        out['shapes'][form] = {}
+        out['timings'][form] = {}
+        old = get_launch_shape(form)
+        for name, _, _, dims in CLASSES:
+            ells = geometry.car_body(dims, 7, count=8,
+                                     box=((0.2 * dims[0], 0.5 * dims[0]), (1, 0.3 * dims[1]),
+                                          (0.35 * dims[2], 0.65 * dims[2])),
+                                     axes=(0.03 * dims[1], 0.12 * dims[1]))
+            mask = geometry.ellipsoids(ells, dims, ground=True)[0]
+            t = Tunnel(dims, form)
+            t.init(0.05)
+            t.set_solid(mask)
+            best, res = None, {}
+            for shape in SHAPES:
+                set_launch_shape(form, *shape)
+                _, r = step_timing(form, dims, mask, steps=steps, warmup=5, tunnel=t)
+                res['%d,%d' % shape] = r['ms_median']
+                if best is None or r['ms_median'] < res['%d,%d' % best]:
+                    best = shape
+            out['shapes'][form][name] = list(best)
+            out['timings'][form][name] = res
+            print(json.dumps({'form': form, 'class': name, 'dims': dims, 'best': best, 'ms': res}), flush=True)
+            del t
+            torch.cuda.empty_cache()
+        set_launch_shape(form, *old)
+    return out
+
+
+def main(argv=None):
+    ap = argparse.ArgumentParser()
+    ap.add_argument('--forms', default='fp32,fp16,bf16,fp64')
+    ap.add_argument('--steps', type=int, default=40)
+    ap.add_argument('--out', default=TUNING_PATH)
+    args = ap.parse_args(argv)
+    res = tune(args.forms.split(','), args.steps)
+    with open(args.out, 'w') as fh:
+        json.dump(res, fh, indent=1)
+    return 0
+
+
+if __name__ == '__main__':
+    sys.exit(main())

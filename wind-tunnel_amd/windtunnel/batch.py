@@ -11,8 +11,9 @@ import ctypes
 
 import torch
 
-from . import _lib
+from . import _lib, autotune
 from ._lib import BatchInfo, Grid, Params, check, stream_handle
+from .tunnel import apply_shape
 
 
 class Batch:
@@ -29,6 +30,8 @@ class Batch:
         nt = len(self.tunnels)
         nbytes = check(self.lib.wt_batch_workspace_bytes(nt), 'wt_batch_workspace_bytes')
         self.workspace = torch.empty(nbytes, dtype=torch.uint8, device=self.tunnels[0].device)
+        # force x, y, z and mass of every tunnel after a step of parity 0 / 1
+        self.sums = torch.zeros((2, nt, 4), dtype=torch.float64, device=self.tunnels[0].device)
         self.info = BatchInfo()
         self.parity = 0
         self.graph = None
@@ -36,10 +39,14 @@ class Batch:
 
     def prepare(self, stream=None):
         nt = len(self.tunnels)
+        # the batch launches with the tuned shape of its largest tunnel's size class
+        big = max(self.tunnels, key=lambda t: t.n)
+        apply_shape(big.form, autotune.shape_for(big.form_name, big.n))
         grids = (Grid * nt)(*[t.grid for t in self.tunnels])
         prm = (Params * nt)(*[Params(*p) for p in self.params])
         cur = (ctypes.c_int32 * nt)(*[t.cur for t in self.tunnels])
-        check(self.lib.wt_batch_prepare(grids, prm, cur, nt, ctypes.c_void_p(self.workspace.data_ptr()),
+        check(self.lib.wt_batch_prepare(grids, prm, cur, nt, ctypes.c_void_p(self.sums.data_ptr()),
+                                        ctypes.c_void_p(self.workspace.data_ptr()),
                                         ctypes.byref(self.info), stream_handle(stream)), 'wt_batch_prepare')
         self.parity = 0
         self.graph = None
@@ -47,6 +54,10 @@ class Batch:
     @property
     def cells(self):
         return sum(t.n for t in self.tunnels)
+
+    def last_sums(self):
+        """Device view (ntunnels, 4) of the sums written by the most recent step."""
+        return self.sums[self.parity ^ 1]
 
     def step(self, flags=0, stream=None):
         """One lattice step of every tunnel (asynchronous)."""

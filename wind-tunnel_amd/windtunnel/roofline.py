@@ -52,6 +52,7 @@ def c2_mask(dims=(512, 256, 256), seed=2025):
 
 def step_timing(form, dims, mask, steps=100, warmup=10, flags=0, tunnel=None):
     t = tunnel or Tunnel(dims, form)
+    t.shape = None          # time whatever launch shape is currently set
     if tunnel is None:
         t.init(0.05)
         t.set_solid(mask)

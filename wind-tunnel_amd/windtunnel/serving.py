@@ -1,0 +1,278 @@
+"""Continuous-batching server for many wind-tunnel sessions on one GPU.
+
+The reference runs one tunnel in a solver thread (server/app.py:50-202): drain
+commands, step, read back a frame, pace. Here that thread becomes a request
+loop that keeps many tunnels of uneven size resident in HBM:
+
+* arrival -> waiting queue (FIFO); geometry for waiting sessions is built on a
+  host thread pool while the GPU works;
+* admission: the head of the queue is admitted as soon as its device footprint
+  (exact bytes of its state, the generalisation of app.py:72-76) fits in the
+  memory budget; sessions arriving while others run join at the next tick;
+* one tick = one lattice step of every active session: one batched launch per
+  storage form (all tunnels of that form in a single kernel launch, or a
+  captured graph replay of two steps), forces of every session reduced in
+  FP64 and fetched with one copy per form, slices every `slice_every` steps;
+* the host keeps at most one tick in flight beyond the one it retires, so
+  results stream back while the next tick runs;
+* a session that reaches its step count is retired and its memory freed.
+
+Each session's arithmetic is independent of the mix it runs in: the batched
+kernel computes every cell exactly as a single-tunnel launch does and the FP64
+reductions use a fixed order, so a session's forces are bit-identical alone
+and in any mix.
+"""
+import collections
+import concurrent.futures
+import dataclasses
+import time
+
+import numpy as np
+import torch
+
+from . import geometry
+from ._lib import WT_FLAG_FORCES
+from .batch import Batch
+from .session import FIELD_RANGES
+from .tunnel import Tunnel, state_bytes
+
+
+@dataclasses.dataclass
+class Request:
+    sid: int
+    arrival: float            # seconds after the start of the run
+    dims: tuple
+    ellipsoids: np.ndarray    # rows cx, cy, cz, ax, ay, az (lattice units)
+    form: str = 'fp32'
+    steps: int = 1000
+    uin: float = 0.05
+    tau: float = 0.52
+    smag: float = 0.16
+    ground: bool = True
+    slice_every: int = 20
+
+
+@dataclasses.dataclass
+class Live:
+    req: Request
+    tunnel: Tunnel
+    admitted: float
+    frontal: int
+    done_steps: int = 0
+    issued_steps: int = 0
+    last_done: float = 0.0
+    forces: list = dataclasses.field(default_factory=list)
+    slices: int = 0
+    slice_dev: object = None
+    slice_host: object = None
+
+
+class Server:
+    def __init__(self, budget_bytes=None, use_graphs=False, keep_forces=True, geometry_workers=4,
+                 device='cuda'):
+        self.device = torch.device(device)
+        if budget_bytes is None:
+            total = torch.cuda.get_device_properties(self.device).total_memory
+            budget_bytes = int(total * 0.85)          # headroom rule of app.py:76
+        self.budget = int(budget_bytes)
+        self.use_graphs = use_graphs
+        self.keep_forces = keep_forces
+        self.pool = concurrent.futures.ThreadPoolExecutor(max_workers=geometry_workers)
+        self.reset_stats()
+
+    def reset_stats(self):
+        self.latencies = []       # seconds between consecutive results of a session
+        self.queue_delays = []    # admission time - arrival time
+        self.cell_steps = 0
+        self.ticks = 0
+        self.peak_bytes = 0
+        self.finished = {}
+
+    # ---- geometry -----------------------------------------------------------------
+    @staticmethod
+    def _geometry(req):
+        mask, counts = geometry.ellipsoids(req.ellipsoids, req.dims, ground=req.ground)
+        return mask, counts
+
+    # ---- the loop -----------------------------------------------------------------
+    def run(self, requests, time_limit=None, clock=time.perf_counter, progress=None):
+        """Serve `requests` (arrival times relative to the start). Returns stats."""
+        pending = collections.deque(sorted(requests, key=lambda r: r.arrival))
+        waiting = collections.deque()
+        futures = {}
+        active = {}                       # form -> list[Live]
+        batches = {}                      # form -> Batch
+        dirty = set()
+        used = 0
+        inflight = collections.deque()    # (event, [(live, step_index)], {form: host sums})
+        t0 = clock()
+        last_progress = t0
+
+        def now():
+            return clock() - t0
+
+        def retire(entry):
+            ev, members, host_sums = entry
+            ev.synchronize()
+            t = now()
+            for form, lives in members.items():
+                hs = host_sums.get(form)
+                hs = hs.numpy() if hs is not None else None
+                for k, lv in enumerate(lives):
+                    lv.done_steps += 1
+                    self.latencies.append(t - lv.last_done)
+                    lv.last_done = t
+                    if self.keep_forces and hs is not None:
+                        lv.forces.append(hs[k, :3].copy())
+            self.ticks += 1
+
+        while pending or waiting or any(active.values()) or inflight:
+            if time_limit is not None and now() > time_limit:
+                break
+            t = now()
+            while pending and pending[0].arrival <= t:
+                r = pending.popleft()
+                waiting.append(r)
+                futures[r.sid] = self.pool.submit(self._geometry, r)
+            # admission: FIFO, the head must fit in the budget
+            while waiting:
+                r = waiting[0]
+                need = state_bytes(r.dims, r.form) + 4 * int(np.prod(r.dims))
+                if used + need > self.budget or not futures[r.sid].done():
+                    break
+                waiting.popleft()
+                mask, counts = futures.pop(r.sid).result()
+                tun = Tunnel(r.dims, r.form, self.device)
+                tun.init(r.uin)
+                tun.set_solid(mask)
+                lv = Live(r, tun, now(), max(1, counts['frontal']))
+                lv.last_done = lv.admitted
+                nx, ny, nz = r.dims
+                lv.slice_dev = torch.zeros(2 * nx * ny, dtype=torch.uint8, device=self.device)
+                lv.slice_host = torch.zeros(2 * nx * ny, dtype=torch.uint8).pin_memory()
+                used += need
+                self.peak_bytes = max(self.peak_bytes, used)
+                self.queue_delays.append(lv.admitted - r.arrival)
+                active.setdefault(r.form, []).append(lv)
+                dirty.add(r.form)
+            if not any(active.values()):
+                while inflight:
+                    retire(inflight.popleft())
+                if not pending and not waiting:
+                    break
+                time.sleep(0.0005)
+                continue
+            # (re)build the batch of every form whose membership changed
+            for form in list(dirty):
+                lives = active.get(form, [])
+                while inflight:              # old descriptors stay valid until retired
+                    retire(inflight.popleft())
+                if lives:
+                    bt = Batch([lv.tunnel for lv in lives], [(lv.req.uin, lv.req.tau, lv.req.smag) for lv in lives])
+                    if self.use_graphs:
+                        bt.capture(WT_FLAG_FORCES)
+                    bt.ring = [torch.zeros((2, len(lives), 4), dtype=torch.float64).pin_memory() for _ in range(4)]
+                    bt.ring_pos = 0
+                    batches[form] = bt
+                else:
+                    batches.pop(form, None)
+                dirty.discard(form)
+            # issue one tick: every active session advances one step, or two when
+            # every form can replay its captured two-step graph
+            groups = [(form, lives, batches[form]) for form, lives in active.items() if lives]
+            two = self.use_graphs and all(
+                b.graph is not None and b.parity == 0 and min(lv.req.steps - lv.issued_steps for lv in lives) >= 2
+                for _, lives, b in groups)
+            nsteps = 2 if two else 1
+            results = [dict() for _ in range(nsteps)]
+            members = {}
+            for form, lives, b in groups:
+                if two:
+                    b.replay()
+                else:
+                    b.step(WT_FLAG_FORCES)
+                for lv in lives:
+                    lv.issued_steps += nsteps
+                    self.cell_steps += nsteps * lv.tunnel.n
+                hs = b.ring[b.ring_pos]
+                b.ring_pos = (b.ring_pos + 1) % len(b.ring)
+                hs.copy_(b.sums, non_blocking=True)
+                if two:
+                    results[0][form], results[1][form] = hs[0], hs[1]
+                else:
+                    results[0][form] = hs[b.parity ^ 1]
+                members[form] = list(lives)   # snapshot: admission mutates `lives`
+                # slices of the sessions whose step count crossed a multiple of slice_every
+                for lv in lives:
+                    se = lv.req.slice_every
+                    if se and (lv.issued_steps // se) > ((lv.issued_steps - nsteps) // se):
+                        nx, ny, nz = lv.req.dims
+                        vmin, vmax = FIELD_RANGES[0]
+                        lv.tunnel.slice(lv.slice_dev, 2, nz // 2, 0, lv.req.uin, 40.0, vmin, vmax)
+                        lv.slice_host.copy_(lv.slice_dev, non_blocking=True)
+                        lv.slices += 1
+            ev = torch.cuda.Event()
+            ev.record()
+            for r in results:
+                inflight.append((ev, members, r))
+            while len(inflight) > 2 * nsteps:
+                retire(inflight.popleft())
+            # retire finished sessions once their last tick is done
+            for form, lives in active.items():
+                fin = [lv for lv in lives if lv.issued_steps >= lv.req.steps]
+                if not fin:
+                    continue
+                while inflight:
+                    retire(inflight.popleft())
+                for lv in fin:
+                    lives.remove(lv)
+                    used -= state_bytes(lv.req.dims, lv.req.form) + 4 * int(np.prod(lv.req.dims))
+                    self.finished[lv.req.sid] = lv
+                    lv.tunnel = None
+                    lv.slice_dev = None
+                dirty.add(form)
+            if progress and clock() - last_progress > progress[0]:
+                last_progress = clock()
+                progress[1](self, now(), sum(len(v) for v in active.values()), len(waiting), len(pending))
+        while inflight:
+            retire(inflight.popleft())
+        wall = now()
+        torch.cuda.synchronize()
+        return self.stats(wall)
+
+    def stats(self, wall):
+        lat = np.array(self.latencies) if self.latencies else np.zeros(1)
+        qd = np.array(self.queue_delays) if self.queue_delays else np.zeros(1)
+        return {
+            'wall_s': wall, 'sessions_done': len(self.finished), 'ticks': self.ticks,
+            'cell_steps': self.cell_steps, 'mlups': self.cell_steps / max(wall, 1e-9) / 1e6,
+            'step_latency_ms': {'p50': float(np.percentile(lat, 50) * 1e3), 'p99': float(np.percentile(lat, 99) * 1e3),
+                                'max': float(lat.max() * 1e3)},
+            'queue_delay_s': {'p50': float(np.percentile(qd, 50)), 'p99': float(np.percentile(qd, 99)),
+                              'max': float(qd.max())},
+            'peak_resident_gb': self.peak_bytes / 1e9,
+        }
+
+
+def mixed_workload(seed=0, duration=600.0, rate=2.0, min_cells=0.5e6, max_cells=33e6, steps=(200, 5000),
+                   forms=('fp32', 'fp16'), max_sessions=None):
+    """Poisson arrivals of sessions with log-uniform grid sizes (shape 2m x m x m,
+    m a multiple of 8), seeded ellipsoid-union bodies on the ground, uniform
+    session lengths, storage forms alternating."""
+    rng = np.random.default_rng(seed)
+    t, out = 0.0, []
+    while True:
+        t += rng.exponential(1.0 / rate)
+        if t > duration or (max_sessions and len(out) >= max_sessions):
+            break
+        cells = float(np.exp(rng.uniform(np.log(min_cells), np.log(max_cells))))
+        m = max(16, int(round((cells / 2) ** (1 / 3) / 8)) * 8)
+        dims = (2 * m, m, m)
+        k = int(rng.integers(3, 9))
+        ells = np.column_stack([rng.uniform(0.2 * 2 * m, 0.45 * 2 * m, k), rng.uniform(1, 0.3 * m, k),
+                                rng.uniform(0.35 * m, 0.65 * m, k), rng.uniform(0.03 * m, 0.12 * m, k),
+                                rng.uniform(0.03 * m, 0.12 * m, k), rng.uniform(0.03 * m, 0.12 * m, k)])
+        out.append(Request(len(out), t, dims, ells, forms[len(out) % len(forms)],
+                           int(rng.integers(steps[0], steps[1] + 1)), float(rng.uniform(0.03, 0.08)),
+                           float(rng.uniform(0.51, 0.60)), 0.16, True, 20))
+    return out

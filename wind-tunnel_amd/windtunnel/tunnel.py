@@ -11,7 +11,7 @@ import ctypes
 import numpy as np
 import torch
 
-from . import _lib
+from . import _lib, autotune
 from ._lib import Grid, Params, SeedBox, check, stream_handle, FORMS, FORM_NAMES
 
 ELEM_DTYPE = {0: torch.float32, 1: torch.int16, 2: torch.int16, 3: torch.float64}
@@ -62,6 +62,8 @@ class Tunnel:
                          self.partials.data_ptr(), self.sums.data_ptr())
         self.gp = ctypes.byref(self.grid)
         self.cur = 0
+        # launch shape of this tunnel's size class (None: keep the library's current shape)
+        self.shape = autotune.shape_for(self.form_name, self.n)
 
     @property
     def form_name(self):
@@ -87,6 +89,8 @@ class Tunnel:
 
     # ---- hot path ----------------------------------------------------------
     def step(self, nsteps, uin, tau, smag, flags=0, stream=None):
+        if self.shape is not None:
+            apply_shape(self.form, self.shape)
         p = Params(float(uin), float(tau), float(smag))
         check(self.lib.wt_step(self.gp, self.cur, int(nsteps), ctypes.byref(p), int(flags),
                                stream_handle(stream)), 'wt_step')
@@ -142,9 +146,20 @@ class Tunnel:
                              float(cp_range), stream_handle(stream)), 'wt_cp')
 
 
+_applied = {}
+
+
 def set_launch_shape(form, cpt, remap):
     form = FORMS[form] if isinstance(form, str) else int(form)
     check(_lib.load().wt_set_launch_shape(form, int(cpt), int(remap)), 'wt_set_launch_shape')
+    _applied[form] = (int(cpt), int(remap))
+
+
+def apply_shape(form, shape):
+    """Set the library's launch shape for `form` unless it is already set."""
+    form = FORMS[form] if isinstance(form, str) else int(form)
+    if _applied.get(form) != tuple(shape):
+        set_launch_shape(form, *shape)
 
 
 def get_launch_shape(form):

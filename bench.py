@@ -69,8 +69,7 @@ def run_gpu(form, steps, warmup, mask, counts, lref):
         s.T.step(1, s.uin, s.tau, s.smag, flags=1 if s.is_frame() else 0)
         ev[k][1].record(stream)
         if s.num_p:
-            s.seed += 1
-            s.T.advect(s.pos, s.spd, 1.0, s.uin, s.max_age, s.seed, s.box)
+            s.advect()
         if s.is_frame():
             s.emit_frame()
         s.step_count += 1

@@ -142,6 +142,19 @@ int wt_get_launch_shape(int form, int* cpt, int* remap);
  * pos: float4 (x, y, z, age) per particle; spd: float per particle. */
 int wt_advect(const wt_grid* g, float* pos, float* spd, int64_t num_p, float dt, float uin,
               float max_age, uint32_t seed, const wt_seed_box* box, void* stream);
+/* Same as wt_advect, with lane j of the launch taking particle order[j]
+ * (a permutation of 0..num_p-1, e.g. from wt_particle_order) so that a wave
+ * samples neighbouring cells. Each particle's arithmetic and random stream
+ * (keyed by its id) are unchanged, so results equal wt_advect's bit for bit. */
+int wt_advect_ordered(const wt_grid* g, float* pos, float* spd, const uint32_t* order, int64_t num_p,
+                      float dt, float uin, float max_age, uint32_t seed, const wt_seed_box* box,
+                      void* stream);
+/* Device workspace bytes for wt_particle_order. */
+int64_t wt_particle_order_workspace_bytes(int64_t num_p);
+/* order[] = particle ids sorted by the index of the cell they are in (radix
+ * sort on the device). No counterpart in the reference (a locality aid). */
+int wt_particle_order(const wt_grid* g, const float* pos, int64_t num_p, uint32_t* order, void* workspace,
+                      int64_t workspace_bytes, void* stream);
 /* Quantize particles (pack_particles, lbm.py:266-275). */
 int wt_pack_particles(const wt_grid* g, const float* pos, const float* spd, uint16_t* q16,
                       uint8_t* q8, int64_t num_p, void* stream);

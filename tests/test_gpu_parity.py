@@ -189,6 +189,35 @@ def test_advect_bit_exact(gpu):
     assert _raw_equal(gp.cpu().numpy(), cp_) and _raw_equal(gs.cpu().numpy(), cs_)
 
 
+def test_ordered_advect_equals_plain(gpu):
+    """Processing particles in cell order changes nothing but speed."""
+    t, mask = _flow_state()
+    rng = np.random.default_rng(9)
+    N = 30001
+    pos = np.column_stack([rng.uniform(-2, t.nx + 2, N), rng.uniform(-2, t.ny + 2, N), rng.uniform(-2, t.nz + 2, N),
+                           rng.uniform(0, 90, N)]).astype(np.float32)
+    pos[5] = np.nan
+    spd = np.ones(N, np.float32)
+    box = (2.0, 10.0, 11.0, 6.0, 5.0)
+    a_pos, a_spd = torch.from_numpy(pos.copy()).cuda(), torch.from_numpy(spd.copy()).cuda()
+    b_pos, b_spd = torch.from_numpy(pos.copy()).cuda(), torch.from_numpy(spd.copy()).cuda()
+    order = torch.empty(N, dtype=torch.int32, device='cuda')
+    ws = t.particle_order(b_pos, order)
+    o = order.cpu().numpy()
+    assert np.array_equal(np.sort(o), np.arange(N))                  # a permutation
+    p = np.nan_to_num(pos[:, :3], nan=0.0)
+    keys = (np.clip(p[:, 0], 0, t.nx - 1).astype(int) + t.nx * (np.clip(p[:, 1], 0, t.ny - 1).astype(int) +
+            t.ny * np.clip(p[:, 2], 0, t.nz - 1).astype(int)))
+    assert np.all(np.diff(keys[o]) >= 0)                             # sorted by cell index
+    for k in range(5):
+        t.advect(a_pos, a_spd, 2.0, 0.05, 60.0, 70 + k, box)
+        if k == 2:
+            ws = t.particle_order(b_pos, order, ws)
+        t.advect_ordered(b_pos, b_spd, order, 2.0, 0.05, 60.0, 70 + k, box)
+    assert _raw_equal(a_pos.cpu().numpy(), b_pos.cpu().numpy())
+    assert _raw_equal(a_spd.cpu().numpy(), b_spd.cpu().numpy())
+
+
 def test_pack_bit_exact(gpu):
     t, _ = _flow_state(steps=2)
     rng = np.random.default_rng(5)

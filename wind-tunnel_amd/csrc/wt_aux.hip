@@ -7,6 +7,8 @@
 // IEEE operations on the same inputs.
 #include <hip/hip_runtime.h>
 
+#include <hipcub/device/device_radix_sort.hpp>
+
 #include <cstring>
 
 #include "wt_device.h"
@@ -78,6 +80,7 @@ __device__ __forceinline__ float rng_f(uint32_t& st) {
 struct AdvectArgs {
   const float4* mac;
   const uint8_t* cls;
+  const uint32_t* order;  // processing order (a permutation of particle ids) or null
   float4* pos;
   float* spd;
   uint32_t num_p;
@@ -89,8 +92,10 @@ struct AdvectArgs {
 
 // lbm.py:233-263
 __global__ __launch_bounds__(ABLOCK) void k_advect(AdvectArgs a) {
-  const uint32_t i = blockIdx.x * ABLOCK + threadIdx.x;
-  if (i >= a.num_p) return;
+  const uint32_t j = blockIdx.x * ABLOCK + threadIdx.x;
+  if (j >= a.num_p) return;
+  // particle id: lanes of a wave take spatial neighbours when an order is given
+  const uint32_t i = a.order ? a.order[j] : j;
   const float4 p = a.pos[i];
   float qx = p.x, qy = p.y, qz = p.z;
   const float4 m1 = sample_mac(a.mac, qx, qy, qz, a.nx, a.ny, a.nz);
@@ -119,6 +124,19 @@ __global__ __launch_bounds__(ABLOCK) void k_advect(AdvectArgs a) {
   }
   a.pos[i] = make_float4(qx, qy, qz, age);
   a.spd[i] = s;
+}
+
+// Sort key of each particle: the index of the cell its position falls in
+// (clamped), so sorted order walks the tunnel in memory order.
+__global__ __launch_bounds__(ABLOCK) void k_particle_keys(const float4* __restrict__ pos, uint32_t* keys,
+                                                          uint32_t* ids, uint32_t num_p, int nx, int ny, int nz) {
+  const uint32_t i = blockIdx.x * ABLOCK + threadIdx.x;
+  if (i >= num_p) return;
+  const float4 p = pos[i];
+  const float qx = clampf(p.x, 0.0f, (float)nx - 1.0f), qy = clampf(p.y, 0.0f, (float)ny - 1.0f);
+  const float qz = clampf(p.z, 0.0f, (float)nz - 1.0f);
+  keys[i] = cidx((int)qx, (int)qy, (int)qz, nx, ny, nz);   // NaN -> 0 after the clamps
+  ids[i] = i;
 }
 
 // lbm.py:266-275
@@ -287,11 +305,68 @@ int wt_advect(const wt_grid* g, float* pos, float* spd, int64_t num_p, float dt,
   if (num_p < 0 || num_p >= ((int64_t)1 << 31)) return fail(-1, "bad particle count %lld", (long long)num_p);
   if (num_p == 0) return 0;
   if (!pos || !spd || !box) return fail(-1, "null particle buffer or seed box");
-  AdvectArgs a{reinterpret_cast<const float4*>(g->mac), g->cls, reinterpret_cast<float4*>(pos), spd,
+  AdvectArgs a{reinterpret_cast<const float4*>(g->mac), g->cls, nullptr, reinterpret_cast<float4*>(pos), spd,
                (uint32_t)num_p, g->nx, g->ny, g->nz, dt, uin, max_age, seed, *box};
   hipLaunchKernelGGL(k_advect, dim3((unsigned)((num_p + ABLOCK - 1) / ABLOCK)), dim3(ABLOCK), 0,
                      (hipStream_t)stream, a);
   return check_launch("wt_advect");
+}
+
+int wt_advect_ordered(const wt_grid* g, float* pos, float* spd, const uint32_t* order, int64_t num_p, float dt,
+                      float uin, float max_age, uint32_t seed, const wt_seed_box* box, void* stream) {
+  if (int e = check_small_grid(g)) return e;
+  if (num_p < 0 || num_p >= ((int64_t)1 << 31)) return fail(-1, "bad particle count %lld", (long long)num_p);
+  if (num_p == 0) return 0;
+  if (!pos || !spd || !box || !order) return fail(-1, "null particle buffer, order or seed box");
+  AdvectArgs a{reinterpret_cast<const float4*>(g->mac), g->cls, order, reinterpret_cast<float4*>(pos), spd,
+               (uint32_t)num_p, g->nx, g->ny, g->nz, dt, uin, max_age, seed, *box};
+  hipLaunchKernelGGL(k_advect, dim3((unsigned)((num_p + ABLOCK - 1) / ABLOCK)), dim3(ABLOCK), 0,
+                     (hipStream_t)stream, a);
+  return check_launch("wt_advect_ordered");
+}
+
+// Workspace: keys, sorted keys, ids (3 x num_p uint32, each 256-byte aligned), then radix-sort scratch.
+static size_t align256(size_t b) { return (b + 255) & ~(size_t)255; }
+
+int64_t wt_particle_order_workspace_bytes(int64_t num_p) {
+  if (num_p <= 0 || num_p >= ((int64_t)1 << 31)) {
+    fail(-1, "bad particle count %lld", (long long)num_p);
+    return -1;
+  }
+  size_t tmp = 0;
+  if (hipcub::DeviceRadixSort::SortPairs(nullptr, tmp, (const uint32_t*)nullptr, (uint32_t*)nullptr,
+                                         (const uint32_t*)nullptr, (uint32_t*)nullptr, (int)num_p) != hipSuccess) {
+    fail(-2, "radix sort size query failed");
+    return -2;
+  }
+  return (int64_t)(3 * align256(4 * (size_t)num_p) + align256(tmp));
+}
+
+int wt_particle_order(const wt_grid* g, const float* pos, int64_t num_p, uint32_t* order, void* workspace,
+                      int64_t workspace_bytes, void* stream) {
+  if (int e = check_small_grid(g)) return e;
+  if (!pos || !order || !workspace) return fail(-1, "null particle, order or workspace buffer");
+  const int64_t need = wt_particle_order_workspace_bytes(num_p);
+  if (need < 0) return (int)need;
+  if (workspace_bytes < need) return fail(-1, "workspace of %lld bytes, need %lld", (long long)workspace_bytes,
+                                          (long long)need);
+  hipStream_t s = (hipStream_t)stream;
+  const size_t a = align256(4 * (size_t)num_p);
+  char* w = (char*)workspace;
+  uint32_t* keys = (uint32_t*)w;
+  uint32_t* keys_sorted = (uint32_t*)(w + a);
+  uint32_t* ids = (uint32_t*)(w + 2 * a);
+  void* tmp = w + 3 * a;
+  size_t tmp_bytes = (size_t)(workspace_bytes - (int64_t)(3 * a));
+  hipLaunchKernelGGL(k_particle_keys, dim3((unsigned)((num_p + ABLOCK - 1) / ABLOCK)), dim3(ABLOCK), 0, s,
+                     reinterpret_cast<const float4*>(pos), keys, ids, (uint32_t)num_p, g->nx, g->ny, g->nz);
+  if (int e = check_launch("wt_particle_order")) return e;
+  int bits = 1;
+  while (bits < 32 && ((uint64_t)1 << bits) < (uint64_t)g->nx * g->ny * g->nz) bits++;
+  if (hipcub::DeviceRadixSort::SortPairs(tmp, tmp_bytes, keys, keys_sorted, ids, order, (int)num_p, 0, bits, s) !=
+      hipSuccess)
+    return fail(-2, "radix sort failed: %s", hipGetErrorString(hipGetLastError()));
+  return check_launch("wt_particle_order");
 }
 
 int wt_pack_particles(const wt_grid* g, const float* pos, const float* spd, uint16_t* q16, uint8_t* q8,

@@ -66,6 +66,9 @@ SIGNATURES = {
     'wt_set_launch_shape': (c_i32, [c_i32, c_i32, c_i32]),
     'wt_get_launch_shape': (c_i32, [c_i32, P(c_i32), P(c_i32)]),
     'wt_advect': (c_i32, [P(Grid), c_vp, c_vp, c_i64, c_f32, c_f32, c_f32, c_u32, P(SeedBox), c_vp]),
+    'wt_advect_ordered': (c_i32, [P(Grid), c_vp, c_vp, c_vp, c_i64, c_f32, c_f32, c_f32, c_u32, P(SeedBox), c_vp]),
+    'wt_particle_order_workspace_bytes': (c_i64, [c_i64]),
+    'wt_particle_order': (c_i32, [P(Grid), c_vp, c_i64, c_vp, c_vp, c_i64, c_vp]),
     'wt_pack_particles': (c_i32, [P(Grid), c_vp, c_vp, c_vp, c_vp, c_i64, c_vp]),
     'wt_slice': (c_i32, [P(Grid), c_vp, c_i32, c_i32, c_i32, c_f32, c_f32, c_f32, c_f32, c_vp]),
     'wt_cp': (c_i32, [P(Grid), c_vp, c_vp, c_vp, c_i64, c_f32, c_f32, c_vp]),

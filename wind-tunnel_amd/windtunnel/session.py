@@ -55,6 +55,10 @@ class Session:
             d[:, 3] = rng.random(self.num_p) * self.max_age * 0.5
             self.pos = torch.from_numpy(d).to(device)
             self.spd = torch.ones(self.num_p, dtype=torch.float32, device=device)
+            # particles are advected in cell order, refreshed every sort_every steps
+            self.order = torch.empty(self.num_p, dtype=torch.int32, device=device)
+            self.order_ws = None
+            self.sort_every = 50
         # slice planes: (axis, index, field)
         self.slices = [tuple(s) for s in slices]
         self.slice_dev, self.slice_host = [], []
@@ -79,13 +83,20 @@ class Session:
         want_force = frame if force is None else force
         self.T.step(1, self.uin, self.tau, self.smag, flags=WT_FLAG_FORCES if want_force else 0, stream=stream)
         if self.num_p:
-            self.seed += 1
-            self.T.advect(self.pos, self.spd, 1.0, self.uin, self.max_age, self.seed, self.box, stream=stream)
+            self.advect(stream)
         if frame:
             self.emit_frame(stream)
         elif want_force:
             self.forces_host.copy_(self.T.sums, non_blocking=True)
         self.step_count += 1
+
+    def advect(self, stream=None):
+        """Particle advection with dt = 1, in the spatial order refreshed every sort_every steps."""
+        if self.step_count % self.sort_every == 0:
+            self.order_ws = self.T.particle_order(self.pos, self.order, self.order_ws, stream=stream)
+        self.seed += 1
+        self.T.advect_ordered(self.pos, self.spd, self.order, 1.0, self.uin, self.max_age, self.seed, self.box,
+                              stream=stream)
 
     def emit_frame(self, stream=None):
         for (axis, index, field), dev, host in zip(self.slices, self.slice_dev, self.slice_host):

@@ -133,6 +133,22 @@ class Tunnel:
                                  float(max_age), int(seed) & 0xffffffff, ctypes.byref(sb),
                                  stream_handle(stream)), 'wt_advect')
 
+    def advect_ordered(self, pos, spd, order, dt, uin, max_age, seed, box, stream=None):
+        sb = SeedBox(*(float(v) for v in box))
+        check(self.lib.wt_advect_ordered(self.gp, _ptr(pos), _ptr(spd), _ptr(order), int(pos.shape[0]), float(dt),
+                                         float(uin), float(max_age), int(seed) & 0xffffffff, ctypes.byref(sb),
+                                         stream_handle(stream)), 'wt_advect_ordered')
+
+    def particle_order(self, pos, order, workspace=None, stream=None):
+        """order[:] = particle ids sorted by cell index; returns the workspace for reuse."""
+        n = int(pos.shape[0])
+        need = check(self.lib.wt_particle_order_workspace_bytes(n), 'wt_particle_order_workspace_bytes')
+        if workspace is None or workspace.numel() < need:
+            workspace = torch.empty(need, dtype=torch.uint8, device=self.device)
+        check(self.lib.wt_particle_order(self.gp, _ptr(pos), n, _ptr(order), _ptr(workspace), workspace.numel(),
+                                         stream_handle(stream)), 'wt_particle_order')
+        return workspace
+
     def pack_particles(self, pos, spd, q16, q8, stream=None):
         check(self.lib.wt_pack_particles(self.gp, _ptr(pos), _ptr(spd), _ptr(q16), _ptr(q8),
                                          int(pos.shape[0]), stream_handle(stream)), 'wt_pack_particles')

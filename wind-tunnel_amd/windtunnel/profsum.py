@@ -1,0 +1,123 @@
+"""Summaries of rocprofv3 output (CSV form) for the measurement records.
+
+    python -m windtunnel.profsum stats <rocprof out dir> [--top 15]
+        per-kernel calls / average / total time from *_kernel_stats.csv
+    python -m windtunnel.profsum traffic <fetch dir> <write dir> --copy-bytes B [--json-dir D]
+        HBM bytes per launch of each kernel from FETCH_SIZE and WRITE_SIZE
+        (*_counter_collection.csv, KiB per dispatch), each scaled by what the
+        same counter reads for wt_stream_copy of B bytes (reads B, writes B)
+
+Prints markdown; with --json-dir also writes step_<form>_<dims>_traffic.json
+files that bench.py reports as roofline.traffic.
+"""
+import argparse
+import collections
+import csv
+import glob
+import json
+import os
+import re
+import sys
+
+BYTES_PER_CELL = {0: 169, 1: 93, 2: 93, 3: 321}
+FORM_NAMES = {0: 'fp32', 1: 'fp16', 2: 'bf16', 3: 'fp64'}
+
+
+def _rows(d, suffix):
+    files = sorted(glob.glob(os.path.join(d, '**', '*' + suffix), recursive=True))
+    if not files:
+        raise SystemExit(f'no *{suffix} under {d}')
+    out = []
+    for f in files:
+        with open(f, newline='') as fh:
+            out += list(csv.DictReader(fh))
+    return out
+
+
+def short(name):
+    name = re.sub(r'\(.*', '', name)
+    return name.replace('void ', '').replace('wt::', '')
+
+
+def stats(d, top=15):
+    rows = _rows(d, 'kernel_stats.csv')
+    rows.sort(key=lambda r: -float(r['TotalDurationNs']))
+    lines = ['| kernel | calls | avg (us) | total (ms) | % |', '|---|---:|---:|---:|---:|']
+    for r in rows[:top]:
+        lines.append(f"| {short(r['Name'])} | {r['Calls']} | {float(r['AverageNs']) / 1e3:.2f} | "
+                     f"{float(r['TotalDurationNs']) / 1e6:.2f} | {float(r['Percentage']):.2f} |")
+    return '\n'.join(lines)
+
+
+def _per_kernel(rows, counter):
+    acc = collections.defaultdict(list)
+    for r in rows:
+        if r.get('Counter_Name') != counter:
+            continue
+        acc[(r['Kernel_Name'], int(r.get('Grid_Size', 0) or 0))].append(float(r['Counter_Value']))
+    return acc
+
+
+def traffic(fetch_dir, write_dir, copy_bytes, json_dir=None, cells=None):
+    fr = _per_kernel(_rows(fetch_dir, 'counter_collection.csv'), 'FETCH_SIZE')
+    wr = _per_kernel(_rows(write_dir, 'counter_collection.csv'), 'WRITE_SIZE')
+    cf = [v for (k, _), vs in fr.items() if 'k_stream_copy' in k for v in vs]
+    cw = [v for (k, _), vs in wr.items() if 'k_stream_copy' in k for v in vs]
+    if not cf or not cw:
+        raise SystemExit('no k_stream_copy dispatch in the counter runs')
+    # median of the copy dispatches: KiB counted for copy_bytes read / written
+    cf.sort(); cw.sort()
+    f_scale = copy_bytes / cf[len(cf) // 2]
+    w_scale = copy_bytes / cw[len(cw) // 2]
+    lines = [f'Scaling: wt_stream_copy of {copy_bytes} B reads FETCH_SIZE {cf[len(cf) // 2]:.0f} and '
+             f'WRITE_SIZE {cw[len(cw) // 2]:.0f} -> {f_scale:.2f} / {w_scale:.2f} bytes per counted unit', '',
+             '| kernel | grid | dispatches | fetch (MB) | write (MB) | total (MB) | B/cell | algorithmic B/cell |',
+             '|---|---:|---:|---:|---:|---:|---:|---:|']
+    out = {}
+    for (name, grid), vs in sorted(fr.items()):
+        if 'k_step' not in name:
+            continue
+        ws = wr.get((name, grid))
+        if not ws:
+            continue
+        fb = sum(vs) / len(vs) * f_scale
+        wb = sum(ws) / len(ws) * w_scale
+        m = re.search(r'k_step<(\d)', name)
+        form = int(m.group(1)) if m else 0
+        n = cells or (grid and grid)  # grid counts threads = cells for cells-per-thread 1
+        per_cell = (fb + wb) / n if n else float('nan')
+        lines.append(f'| {short(name)} | {grid} | {len(vs)} | {fb / 1e6:.1f} | {wb / 1e6:.1f} | {(fb + wb) / 1e6:.1f} | '
+                     f'{per_cell:.1f} | {BYTES_PER_CELL[form]} |')
+        out[FORM_NAMES[form]] = {'kernel': short(name), 'grid': grid, 'fetch_bytes': fb, 'write_bytes': wb,
+                                 'hbm_bytes_per_launch': fb + wb, 'bytes_per_cell': per_cell,
+                                 'algorithmic_bytes_per_cell': BYTES_PER_CELL[form],
+                                 'scale_fetch': f_scale, 'scale_write': w_scale}
+    if json_dir:
+        for form, v in out.items():
+            with open(os.path.join(json_dir, f'step_{form}_512x256x256_traffic.json'), 'w') as fh:
+                json.dump(v, fh, indent=1)
+    return '\n'.join(lines)
+
+
+def main(argv=None):
+    ap = argparse.ArgumentParser()
+    sub = ap.add_subparsers(dest='cmd', required=True)
+    a = sub.add_parser('stats')
+    a.add_argument('dir')
+    a.add_argument('--top', type=int, default=15)
+    b = sub.add_parser('traffic')
+    b.add_argument('fetch_dir')
+    b.add_argument('write_dir')
+    b.add_argument('--copy-bytes', type=int, required=True)
+    b.add_argument('--json-dir', default=None)
+    b.add_argument('--cells', type=int, default=512 * 256 * 256)
+    args = ap.parse_args(argv)
+    if args.cmd == 'stats':
+        print(stats(args.dir, args.top))
+    else:
+        print(traffic(args.fetch_dir, args.write_dir, args.copy_bytes, args.json_dir, args.cells))
+    return 0
+
+
+if __name__ == '__main__':
+    sys.exit(main())

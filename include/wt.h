@@ -132,7 +132,8 @@ int wt_batch_prepare(const wt_grid* grids, const wt_params* params, const int* c
 int wt_batch_step(void* workspace, const wt_batch_info* info, int parity, int flags, void* stream);
 
 /* Launch shape of the step kernels of one storage form: cells per thread
- * (1, 2 or 4) and XCD-contiguous tile order (0/1). Chosen by the autotuner;
+ * (1, 2 or 4) and tile order flags: bit 0 XCD-contiguous tiles, bit 1 sweep
+ * direction alternating with the buffer parity. Chosen by the autotuner;
  * results do not depend on it. Applies to later wt_step / wt_batch_prepare. */
 int wt_set_launch_shape(int form, int cpt, int remap);
 int wt_get_launch_shape(int form, int* cpt, int* remap);

@@ -130,7 +130,7 @@ def test_fused_force_equals_standalone(gpu):
     assert _raw_equal(fused[:3], alone[:3])
 
 
-@pytest.mark.parametrize('cpt,remap', [(1, 0), (2, 1), (4, 1), (4, 0)])
+@pytest.mark.parametrize('cpt,remap', [(1, 0), (2, 1), (4, 1), (4, 0), (1, 2), (2, 3)])
 def test_launch_shape_does_not_change_results(gpu, cpt, remap):
     from windtunnel import tunnel as T
     from windtunnel._lib import WT_FLAG_FORCES, WT_FLAG_MASS

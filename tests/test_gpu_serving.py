@@ -50,3 +50,19 @@ def test_sessions_identical_alone_and_in_the_mix(gpu, graphs):
         want = _alone(r)
         assert np.array_equal(got.view(np.uint64), want.view(np.uint64)), r.sid
         assert lv.slices == r.steps // r.slice_every
+
+
+def test_preemption_keeps_results_exact(gpu):
+    """A session evicted to host memory and resumed produces the same forces."""
+    from windtunnel.serving import Server
+    from windtunnel.tunnel import state_bytes
+    reqs = _requests()
+    biggest = max(state_bytes(r.dims, r.form) + 4 * int(np.prod(r.dims)) for r in reqs)
+    srv = Server(budget_bytes=int(1.6 * biggest), preempt_after=0.0)
+    st = srv.run(reqs)
+    assert st['sessions_done'] == len(reqs)
+    assert st['evictions'] > 0
+    for r in reqs:
+        lv = srv.finished[r.sid]
+        assert lv.done_steps == r.steps
+        assert np.array_equal(np.array(lv.forces).view(np.uint64), _alone(r).view(np.uint64)), r.sid

@@ -37,6 +37,9 @@
 namespace wt {
 
 constexpr int BLOCK = 256;       // threads per workgroup (4 wave64)
+#ifndef WT_STEP_ATTR
+#define WT_STEP_ATTR
+#endif
 constexpr int SLOT = 256;        // cells per reduction slot
 constexpr int CHUNK = 1024;      // slots per first-level reduction block
 
@@ -485,18 +488,23 @@ __device__ __forceinline__ void run_tile(const StepDesc& d, int parity, uint32_t
     slot_flush<CPT>(lds, d.partials, tile * CPT, (d.n + SLOT - 1) / SLOT);
 }
 
-// Workgroups are dispatched to the 8 XCDs round-robin; this maps them so that
-// each XCD walks one contiguous range of tiles (neighbouring tiles share
-// cache lines through the x +- 1 pulls).
+// Tile visiting order. Bit 0: workgroups are dispatched to the 8 XCDs
+// round-robin; map them so each XCD walks one contiguous range of tiles
+// (neighbouring tiles share cache lines through the x +- 1 pulls). Bit 1: walk
+// the tiles backwards, so that a step started right after a forward step
+// first reads what that step wrote last, while it is still in the caches.
 __device__ __forceinline__ uint32_t xcd_tile(uint32_t b, uint32_t nblocks, int remap) {
-  if (!remap) return b;
-  const uint32_t q = nblocks >> 3, r = nblocks & 7u;
-  const uint32_t xcd = b & 7u, k = b >> 3;
-  return xcd * q + (xcd < r ? xcd : r) + k;
+  uint32_t t = b;
+  if (remap & 1) {
+    const uint32_t q = nblocks >> 3, r = nblocks & 7u;
+    const uint32_t xcd = b & 7u, k = b >> 3;
+    t = xcd * q + (xcd < r ? xcd : r) + k;
+  }
+  return (remap & 2) ? nblocks - 1u - t : t;
 }
 
 template <int FORM, int CPT>
-__global__ __launch_bounds__(BLOCK) void k_step(StepDesc d, int parity, int remap, int flags) {
+__global__ __launch_bounds__(BLOCK) WT_STEP_ATTR void k_step(StepDesc d, int parity, int remap, int flags) {
   const uint32_t tile = xcd_tile(blockIdx.x, gridDim.x, remap);
   run_tile<FORM, CPT>(d, parity, tile, flags);
 }
@@ -826,7 +834,7 @@ extern "C" {
 int wt_set_launch_shape(int form, int cpt, int remap) {
   if (form < WT_FP32 || form > WT_FP64) return fail(-1, "unknown storage form %d", form);
   if (cpt != 1 && cpt != 2 && cpt != 4) return fail(-1, "cells per thread must be 1, 2 or 4 (got %d)", cpt);
-  g_shape[form] = Shape{cpt, remap ? 1 : 0};
+  g_shape[form] = Shape{cpt, remap & 3};
   return 0;
 }
 
@@ -908,7 +916,10 @@ int wt_step(const wt_grid* g, int cur, int nsteps, const wt_params* p, int flags
   const StepDesc d = make_desc(g, cur, p, 0);
   for (int k = 0; k < nsteps; k++) {
     const int f = (k == nsteps - 1) ? (flags & (WT_FLAG_FORCES | WT_FLAG_MASS)) : 0;
-    launch_step(g->form, d, k & 1, sh, f, s);
+    // with shape bit 1 the sweep direction alternates with the buffer parity
+    Shape shk = sh;
+    shk.remap = (sh.remap & 1) | (((sh.remap & 2) && ((cur ^ k) & 1)) ? 2 : 0);
+    launch_step(g->form, d, k & 1, shk, f, s);
   }
   if (nsteps > 0 && (flags & (WT_FLAG_FORCES | WT_FLAG_MASS))) {
     if (int e = check_launch("wt_step")) return e;
@@ -1023,11 +1034,12 @@ int wt_batch_step(void* workspace, const wt_batch_info* info, int parity, int fl
   const RedDesc* rd = (const RedDesc*)((const char*)workspace + sizeof(StepDesc) * nt);
   const int f = flags & (WT_FLAG_FORCES | WT_FLAG_MASS);
   const uint32_t ntile = (uint32_t)info->total_tiles;
+  const int remap = (info->remap & 1) | (((info->remap & 2) && (parity & 1)) ? 2 : 0);
   switch (form) {
-    case WT_FP32: launch_batch_f<WT_FP32>(dd, nt, ntile, parity, info->cpt, info->remap, f, s); break;
-    case WT_FP16: launch_batch_f<WT_FP16>(dd, nt, ntile, parity, info->cpt, info->remap, f, s); break;
-    case WT_BF16: launch_batch_f<WT_BF16>(dd, nt, ntile, parity, info->cpt, info->remap, f, s); break;
-    case WT_FP64: launch_batch_f<WT_FP64>(dd, nt, ntile, parity, info->cpt, info->remap, f, s); break;
+    case WT_FP32: launch_batch_f<WT_FP32>(dd, nt, ntile, parity, info->cpt, remap, f, s); break;
+    case WT_FP16: launch_batch_f<WT_FP16>(dd, nt, ntile, parity, info->cpt, remap, f, s); break;
+    case WT_BF16: launch_batch_f<WT_BF16>(dd, nt, ntile, parity, info->cpt, remap, f, s); break;
+    case WT_FP64: launch_batch_f<WT_FP64>(dd, nt, ntile, parity, info->cpt, remap, f, s); break;
   }
   if (f) {
     hipLaunchKernelGGL(k_reduce1, dim3((unsigned)info->max_chunks, (unsigned)nt), dim3(BLOCK), 0, s, rd);

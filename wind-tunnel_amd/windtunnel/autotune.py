@@ -1,7 +1,8 @@
 """Launch-shape autotuner for the fused step kernel.
 
 Shapes: cells per thread (1, 2 or 4 consecutive 256-cell slots per
-workgroup) x tile order (plain or XCD-contiguous). Results do not depend on the
+workgroup) x tile order (bit 0: XCD-contiguous; bit 1: sweep direction
+alternating every step). Results do not depend on the
 shape (the force/mass slots are reduced identically), only the speed does.
 For each storage form and size class the tuner times every shape on a
 representative tunnel and stores the fastest in tuning.json beside this file;
@@ -24,7 +25,7 @@ CLASSES = [
     ('L', 2_000_000, 12_000_000, (256, 128, 128)),
     ('XL', 12_000_000, 1 << 31, (512, 256, 256)),
 ]
-SHAPES = [(1, 0), (1, 1), (2, 0), (2, 1), (4, 0), (4, 1)]
+SHAPES = [(1, 0), (1, 1), (1, 2), (1, 3), (2, 0), (2, 1), (2, 2), (4, 0), (4, 1)]
 DEFAULT = (1, 0)
 
 _table = None

@@ -88,7 +88,7 @@ def main(argv=None):
         print(json.dumps(copy_bandwidth(args.copy_gb)), flush=True)
     mask = c2_mask(dims)
     for form in args.forms.split(','):
-        shapes = [(1, 0), (1, 1), (2, 0), (2, 1), (4, 0), (4, 1)] if args.sweep else [get_launch_shape(form)]
+        shapes = [(1, 0), (1, 1), (1, 2), (1, 3), (2, 0), (2, 2), (4, 0)] if args.sweep else [get_launch_shape(form)]
         t = None
         old = get_launch_shape(form)
         for cpt, remap in shapes:

@@ -65,11 +65,13 @@ class Live:
     slices: int = 0
     slice_dev: object = None
     slice_host: object = None
+    saved: object = None        # host checkpoint while evicted
+    evictions: int = 0
 
 
 class Server:
     def __init__(self, budget_bytes=None, use_graphs=False, keep_forces=True, geometry_workers=4,
-                 device='cuda'):
+                 device='cuda', preempt_after=None):
         self.device = torch.device(device)
         if budget_bytes is None:
             total = torch.cuda.get_device_properties(self.device).total_memory
@@ -77,6 +79,9 @@ class Server:
         self.budget = int(budget_bytes)
         self.use_graphs = use_graphs
         self.keep_forces = keep_forces
+        # evict the session with the most remaining steps to host memory when the
+        # queue head has waited this long for memory (None: never)
+        self.preempt_after = preempt_after
         self.pool = concurrent.futures.ThreadPoolExecutor(max_workers=geometry_workers)
         self.reset_stats()
 
@@ -87,6 +92,7 @@ class Server:
         self.ticks = 0
         self.peak_bytes = 0
         self.finished = {}
+        self.evictions = 0
 
     # ---- geometry -----------------------------------------------------------------
     @staticmethod
@@ -136,23 +142,39 @@ class Server:
                 futures[r.sid] = self.pool.submit(self._geometry, r)
             # admission: FIFO, the head must fit in the budget
             while waiting:
-                r = waiting[0]
+                item = waiting[0]
+                resumed = isinstance(item, Live)
+                r = item.req if resumed else item
                 need = state_bytes(r.dims, r.form) + 4 * int(np.prod(r.dims))
-                if used + need > self.budget or not futures[r.sid].done():
+                if used + need > self.budget:
+                    if self.preempt_after is not None and not resumed and now() - r.arrival > self.preempt_after:
+                        victim = self._pick_victim(active)
+                        if victim is not None:
+                            used -= self._evict(victim, active, inflight, retire, dirty)
+                            waiting.append(victim)
+                            continue
+                    break
+                if not resumed and not futures[r.sid].done():
                     break
                 waiting.popleft()
-                mask, counts = futures.pop(r.sid).result()
-                tun = Tunnel(r.dims, r.form, self.device)
-                tun.init(r.uin)
-                tun.set_solid(mask)
-                lv = Live(r, tun, now(), max(1, counts['frontal']))
-                lv.last_done = lv.admitted
+                if resumed:
+                    lv = item
+                    lv.tunnel = Tunnel.from_checkpoint(lv.saved, self.device)
+                    lv.saved = None
+                else:
+                    mask, counts = futures.pop(r.sid).result()
+                    tun = Tunnel(r.dims, r.form, self.device)
+                    tun.init(r.uin)
+                    tun.set_solid(mask)
+                    lv = Live(r, tun, now(), max(1, counts['frontal']))
+                    lv.last_done = lv.admitted
+                    self.queue_delays.append(lv.admitted - r.arrival)
                 nx, ny, nz = r.dims
                 lv.slice_dev = torch.zeros(2 * nx * ny, dtype=torch.uint8, device=self.device)
-                lv.slice_host = torch.zeros(2 * nx * ny, dtype=torch.uint8).pin_memory()
+                if lv.slice_host is None:
+                    lv.slice_host = torch.zeros(2 * nx * ny, dtype=torch.uint8).pin_memory()
                 used += need
                 self.peak_bytes = max(self.peak_bytes, used)
-                self.queue_delays.append(lv.admitted - r.arrival)
                 active.setdefault(r.form, []).append(lv)
                 dirty.add(r.form)
             if not any(active.values()):
@@ -240,6 +262,27 @@ class Server:
         torch.cuda.synchronize()
         return self.stats(wall)
 
+    @staticmethod
+    def _pick_victim(active):
+        cands = [lv for lives in active.values() for lv in lives]
+        if not cands:
+            return None
+        return max(cands, key=lambda lv: (lv.req.steps - lv.issued_steps) * lv.tunnel.n)
+
+    def _evict(self, lv, active, inflight, retire, dirty):
+        """Checkpoint a session to pinned host memory and free its device state."""
+        while inflight:
+            retire(inflight.popleft())
+        lv.saved = lv.tunnel.checkpoint()
+        torch.cuda.current_stream().synchronize()
+        active[lv.req.form].remove(lv)
+        dirty.add(lv.req.form)
+        lv.tunnel = None
+        lv.slice_dev = None
+        lv.evictions += 1
+        self.evictions += 1
+        return state_bytes(lv.req.dims, lv.req.form) + 4 * int(np.prod(lv.req.dims))
+
     def stats(self, wall):
         lat = np.array(self.latencies) if self.latencies else np.zeros(1)
         qd = np.array(self.queue_delays) if self.queue_delays else np.zeros(1)
@@ -250,7 +293,7 @@ class Server:
                                 'max': float(lat.max() * 1e3)},
             'queue_delay_s': {'p50': float(np.percentile(qd, 50)), 'p99': float(np.percentile(qd, 99)),
                               'max': float(qd.max())},
-            'peak_resident_gb': self.peak_bytes / 1e9,
+            'peak_resident_gb': self.peak_bytes / 1e9, 'evictions': self.evictions,
         }
 
 

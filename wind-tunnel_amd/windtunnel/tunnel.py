@@ -126,6 +126,30 @@ class Tunnel:
             g = (u16.astype(np.uint32) << 16).view(np.float32)
         return (W.astype(np.float32)[:, None] + g).astype(dtype)
 
+    # ---- checkpoint / resume ---------------------------------------------------
+    def checkpoint(self, stream=None):
+        """Copy the state that defines the next step -- f[cur], the class bytes, mac
+        and the face-layer snapshot of the current parity -- into pinned host memory
+        (asynchronous on `stream`; synchronize before reading or freeing the tunnel)."""
+        k = self.cur
+        st = {'dims': (self.nx, self.ny, self.nz), 'form': self.form}
+        for name, t in (('f', self.f[k]), ('cls', self.cls), ('mac', self.mac), ('shell', self.shell[k])):
+            h = torch.empty(t.shape, dtype=t.dtype).pin_memory()
+            h.copy_(t, non_blocking=True)
+            st[name] = h
+        return st
+
+    @classmethod
+    def from_checkpoint(cls_, st, device=None, stream=None):
+        """A new resident tunnel continuing exactly where checkpoint() left off."""
+        t = cls_(st['dims'], st['form'], device)
+        t.f[0].copy_(st['f'], non_blocking=True)
+        t.cls.copy_(st['cls'], non_blocking=True)
+        t.mac.copy_(st['mac'], non_blocking=True)
+        t.shell[0].copy_(st['shell'], non_blocking=True)
+        t.cur = 0
+        return t
+
     # ---- particles, slices, surface -------------------------------------------
     def advect(self, pos, spd, dt, uin, max_age, seed, box, stream=None):
         sb = SeedBox(*(float(v) for v in box))

@@ -53,16 +53,21 @@ def test_sessions_identical_alone_and_in_the_mix(gpu, graphs):
 
 
 def test_preemption_keeps_results_exact(gpu):
-    """A session evicted to host memory and resumed produces the same forces."""
-    from windtunnel.serving import Server
+    """Three equal sessions arrive together and only two fit: the third evicts the
+    one with the most remaining work to host memory; the evicted session later
+    resumes from its checkpoint. Every force stream equals the session run alone."""
+    from windtunnel.serving import Request, Server
     from windtunnel.tunnel import state_bytes
-    reqs = _requests()
-    biggest = max(state_bytes(r.dims, r.form) + 4 * int(np.prod(r.dims)) for r in reqs)
-    srv = Server(budget_bytes=int(1.6 * biggest), preempt_after=0.0)
+    dims = (96, 48, 48)
+    reqs = [Request(k, 0.001 * k, dims, np.array([[30.0, 10.0 + 3 * k, 24.0, 8.0, 6.0, 6.0]]),
+                    'fp32' if k != 1 else 'fp16', 120 + 40 * k, 0.05 + 0.01 * k, 0.52, 0.16, True, 20)
+            for k in range(3)]
+    one = max(state_bytes(dims, r.form) + 4 * int(np.prod(dims)) for r in reqs)
+    srv = Server(budget_bytes=int(2.5 * one), preempt_after=0.0)
     st = srv.run(reqs)
-    assert st['sessions_done'] == len(reqs)
-    assert st['evictions'] > 0
+    assert st['sessions_done'] == 3
+    assert st['evictions'] >= 1 and any(srv.finished[k].evictions for k in range(3))
     for r in reqs:
         lv = srv.finished[r.sid]
-        assert lv.done_steps == r.steps
+        assert lv.done_steps == r.steps and len(lv.forces) == r.steps
         assert np.array_equal(np.array(lv.forces).view(np.uint64), _alone(r).view(np.uint64)), r.sid

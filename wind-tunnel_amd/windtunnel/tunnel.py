@@ -144,6 +144,9 @@ class Tunnel:
         """A new resident tunnel continuing exactly where checkpoint() left off."""
         t = cls_(st['dims'], st['form'], device)
         t.f[0].copy_(st['f'], non_blocking=True)
+        # solid cells are never written by a step, so both buffers hold the same
+        # values there; restoring both keeps the raw state identical
+        t.f[1].copy_(st['f'], non_blocking=True)
         t.cls.copy_(st['cls'], non_blocking=True)
         t.mac.copy_(st['mac'], non_blocking=True)
         t.shell[0].copy_(st['shell'], non_blocking=True)

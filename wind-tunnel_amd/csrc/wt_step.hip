@@ -262,6 +262,33 @@ __device__ __forceinline__ void add_force(const uint8_t* cls, uint32_t idx, int3
   }
 }
 
+// ---- face equilibrium boundaries (lbm.py:84-109) -------------------------------------
+// Boundary density and velocity of a face cell: inlet values at x == 0, else the
+// previous step's values of the nearest interior cell with the wall and outlet
+// overrides.
+template <typename R>
+__device__ __forceinline__ void face_state(const StepDesc& d, const Shell<R>& shin, int x, int y, int z,
+                                           R& br, R& bux, R& buy, R& buz) {
+  const R zero = (R)0, one = (R)1;
+  if (x == 0) {
+    br = one; bux = (R)d.uin; buy = zero; buz = zero;
+    return;
+  }
+  const R* m = shell_read<R>(shin, x, y, z);
+  br = clampv(m[3], (R)0.9, (R)1.1);
+  bux = m[0]; buy = m[1]; buz = m[2];
+  if (y == 0 || y == d.ny - 1) buy = zero;
+  if (z == 0 || z == d.nz - 1) buz = zero;
+  if (x == d.nx - 1) { br = one; bux = maxv(bux, zero); }
+}
+
+// Equilibrium populations of a face cell (lbm.py:104-107).
+template <typename R>
+__device__ __forceinline__ R face_feq(int i, R br, R bux, R buy, R buz, R bsq) {
+  const R bc = (R)3.0 * cdot<R>(i, bux, buy, buz);
+  return (R)wd(i) * br * ((R)1 + bc + (R)0.5 * bc * bc - bsq);
+}
+
 // ---- one cell, general path (any mix of solid, face and near-wall lanes) ------------
 template <int FORM>
 __device__ __forceinline__ void step_cell_general(const StepDesc& d, const typename Form<FORM>::elem* fin,
@@ -284,25 +311,12 @@ __device__ __forceinline__ void step_cell_general(const StepDesc& d, const typen
     return;
   }
   if (face) {  // equilibrium boundaries, lbm.py:84-109
-    const R uin = (R)d.uin;
     R br, bux, buy, buz;
-    if (x == 0) {
-      br = one; bux = uin; buy = zero; buz = zero;
-    } else {
-      const R* m = shell_read<R>(shin, x, y, z);
-      br = clampv(m[3], (R)0.9, (R)1.1);
-      bux = m[0]; buy = m[1]; buz = m[2];
-      if (y == 0 || y == ny - 1) buy = zero;
-      if (z == 0 || z == nz - 1) buz = zero;
-      if (x == nx - 1) { br = one; bux = maxv(bux, zero); }
-    }
+    face_state<R>(d, shin, x, y, z, br, bux, buy, buz);
     const R bsq = (R)1.5 * (bux * bux + buy * buy + buz * buz);
 #pragma unroll
-    for (int i = 0; i < Q; i++) {
-      const R bc = (R)3.0 * cdot<R>(i, bux, buy, buz);
-      const R v = (R)wd(i) * br * (one + bc + (R)0.5 * bc * bc - bsq);
-      st_stream(fout + (size_t)i * n + idx, S::store(v, i));
-    }
+    for (int i = 0; i < Q; i++)
+      st_stream(fout + (size_t)i * n + idx, S::store(face_feq<R>(i, br, bux, buy, buz, bsq), i));
     d.mac[idx] = make_float4((float)bux, (float)buy, (float)buz, (float)br);
     if (flags & WT_FLAG_MASS) acc.m += (double)br;
     return;
@@ -373,8 +387,9 @@ template <> struct RawIO<uint16_t> {
 template <int FORM>
 __device__ __forceinline__ void step_cell_fast(const StepDesc& d, const typename Form<FORM>::elem* fin,
                                                typename Form<FORM>::elem* fout,
+                                               const Shell<typename Form<FORM>::real>& shin,
                                                const Shell<typename Form<FORM>::real>& shout, uint32_t idx,
-                                               int x, int y, int z, int flags, Acc& acc) {
+                                               int x, int y, int z, bool xface, int flags, Acc& acc) {
   typedef Form<FORM> S;
   typedef typename S::real R;
   typedef typename S::elem E;
@@ -382,32 +397,43 @@ __device__ __forceinline__ void step_cell_fast(const StepDesc& d, const typename
   const uint32_t n = d.n;
   const int32_t nxy = nx * d.ny;
   const uint32_t bo = idx * (uint32_t)sizeof(E);
-  R f[Q];
   R out[Q], ux, uy, uz, rho;
-  if constexpr (sizeof(E) <= 4) {
-    const int bytes = (int)(19u * n * (uint32_t)sizeof(E));
-    const __amdgpu_buffer_rsrc_t rin = __builtin_amdgcn_make_buffer_rsrc((void*)fin, (short)0, bytes, BUF_GFX9_WORD3);
-    const __amdgpu_buffer_rsrc_t rout = __builtin_amdgcn_make_buffer_rsrc((void*)fout, (short)0, bytes, BUF_GFX9_WORD3);
+  if (xface) {
+    // inlet / outlet lanes of this wave (lbm.py:84-109)
+    face_state<R>(d, shin, x, y, z, rho, ux, uy, uz);
+    const R bsq = (R)1.5 * (ux * ux + uy * uy + uz * uz);
 #pragma unroll
-    for (int i = 0; i < Q; i++) {
-      const uint32_t so = ((uint32_t)i * n - (uint32_t)offs(i, nx, nxy)) * (uint32_t)sizeof(E);
-      f[i] = S::load(RawIO<E>::ld(rin, bo, so), i);
+    for (int i = 0; i < Q; i++) out[i] = face_feq<R>(i, rho, ux, uy, uz, bsq);
+  } else {
+    R f[Q];
+    if constexpr (sizeof(E) <= 4) {
+      const int bytes = (int)(19u * n * (uint32_t)sizeof(E));
+      const __amdgpu_buffer_rsrc_t rin = __builtin_amdgcn_make_buffer_rsrc((void*)fin, (short)0, bytes, BUF_GFX9_WORD3);
+#pragma unroll
+      for (int i = 0; i < Q; i++) {
+        const uint32_t so = ((uint32_t)i * n - (uint32_t)offs(i, nx, nxy)) * (uint32_t)sizeof(E);
+        f[i] = S::load(RawIO<E>::ld(rin, bo, so), i);
+      }
+    } else {
+#pragma unroll
+      for (int i = 0; i < Q; i++) {
+        const char* plane = (const char*)(fin + (size_t)i * n - (ptrdiff_t)offs(i, nx, nxy));
+        f[i] = S::load(ld_stream((const E*)(plane + bo)), i);
+      }
     }
     collide<R>(f, x, nx, (R)d.tau, (R)d.smag, out, ux, uy, uz, rho);
+  }
+  if constexpr (sizeof(E) <= 4) {
+    const int bytes = (int)(19u * n * (uint32_t)sizeof(E));
+    const __amdgpu_buffer_rsrc_t rout = __builtin_amdgcn_make_buffer_rsrc((void*)fout, (short)0, bytes, BUF_GFX9_WORD3);
 #pragma unroll
     for (int i = 0; i < Q; i++) RawIO<E>::st(rout, bo, (uint32_t)i * n * (uint32_t)sizeof(E), S::store(out[i], i));
   } else {
 #pragma unroll
-    for (int i = 0; i < Q; i++) {
-      const char* plane = (const char*)(fin + (size_t)i * n - (ptrdiff_t)offs(i, nx, nxy));
-      f[i] = S::load(ld_stream((const E*)(plane + bo)), i);
-    }
-    collide<R>(f, x, nx, (R)d.tau, (R)d.smag, out, ux, uy, uz, rho);
-#pragma unroll
     for (int i = 0; i < Q; i++) st_stream((E*)((char*)(fout + (size_t)i * n) + bo), S::store(out[i], i));
   }
   d.mac[idx] = make_float4((float)ux, (float)uy, (float)uz, (float)rho);
-  shell_write<R>(shout, x, y, z, ux, uy, uz, rho);
+  if (!xface) shell_write<R>(shout, x, y, z, ux, uy, uz, rho);
   if (flags & WT_FLAG_MASS) acc.m += (double)rho;
 }
 
@@ -424,10 +450,12 @@ __device__ __forceinline__ void step_cell(const StepDesc& d, const typename Form
   const int y = (int)(yz - zz * (uint32_t)ny);
   const int z = (int)zz;
   const uint8_t c = d.cls[idx];
-  const bool face = (x == 0) | (x == nx - 1) | (y == 0) | (y == ny - 1) | (z == 0) | (z == nz - 1);
-  const bool simple = (c == 0) & !face;
+  const bool xface = (x == 0) | (x == nx - 1);
+  const bool face = xface | (y == 0) | (y == ny - 1) | (z == 0) | (z == nz - 1);
+  // fluid lanes away from walls, and fluid inlet/outlet lanes, take the fast path
+  const bool simple = (c == 0) & (!face | xface) & (y != 0) & (y != ny - 1) & (z != 0) & (z != nz - 1);
   if (__all(simple))
-    step_cell_fast<FORM>(d, fin, fout, shout, idx, x, y, z, flags, acc);
+    step_cell_fast<FORM>(d, fin, fout, shin, shout, idx, x, y, z, xface, flags, acc);
   else
     step_cell_general<FORM>(d, fin, fout, shin, shout, idx, c, x, y, z, face, flags, acc);
 }
@@ -783,7 +811,7 @@ struct Shape {
   int cpt;
   int remap;
 };
-static Shape g_shape[4] = {{1, 1}, {1, 1}, {1, 1}, {1, 1}};
+static Shape g_shape[4] = {{1, 0}, {1, 0}, {1, 0}, {1, 0}};
 
 template <int FORM, int CPT>
 static void launch_step_t(const StepDesc& d, int parity, int remap, int flags, hipStream_t s) {

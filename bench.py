@@ -111,9 +111,32 @@ def cpu_baseline(mask, target_s=12.0):
     t.step(k, UIN, TAU, SMAG)
     dt = time.perf_counter() - t0
     n = t.n
+    # one core on the same tunnel (memory already touched by the run above)
+    O.set_threads(1)
+    t0 = time.perf_counter()
+    t.step(1, UIN, TAU, SMAG)
+    one_core = n / (time.perf_counter() - t0) / 1e6
+    del t
+    # config 1: 128x64x64, sphere D=16 at (40,32,32), uin 0.05, tau 0.52, Cs 0.17, no ground
+    from windtunnel import geometry
+    d1 = (128, 64, 64)
+    m1 = geometry.sphere(d1, (40, 32, 32), 16)[0]
+    c1 = {}
+    for nthr, steps in ((threads, 500), (1, 60)):
+        O.set_threads(nthr)
+        t1 = O.OracleTunnel(d1, O.FP32)
+        t1.init(0.05)
+        t1.set_solid(m1)
+        t0 = time.perf_counter()
+        t1.step(steps, 0.05, 0.52, 0.17)
+        c1[nthr] = (steps, d1[0] * d1[1] * d1[2] * steps / (time.perf_counter() - t0) / 1e6)
+    O.set_threads(threads)
     return {'value': round(n * k / dt / 1e6, 2), 'unit': 'MLUPS', 'cores': threads, 'kind': 'port',
             'sample': f'{k} FP32 lattice steps (no particles/slices) of the same 512x256x256 tunnel and '
-                      f'body, after 1 untimed step; oracle/wt_oracle.cpp with {threads} OpenMP threads'}
+                      f'body, after 1 untimed step; oracle/wt_oracle.cpp with {threads} OpenMP threads',
+            'one_core': round(one_core, 2),
+            'config1_128x64x64': {'mlups_all_cores': round(c1[threads][1], 2), 'steps_all_cores': c1[threads][0],
+                                  'mlups_one_core': round(c1[1][1], 2), 'steps_one_core': c1[1][0]}}
 
 
 def load_traffic(form):
@@ -195,6 +218,7 @@ def main():
             continue
         line[form] = {'value': round(rr['mlups'], 1), 'ratio_vs_fp32': round(rr['mlups'] / r['mlups'], 3),
                       'kernel_ms': round(rr['step_kernel_ms'], 4),
+                      'kernel_ratio_vs_fp32': round(r['step_kernel_ms'] / rr['step_kernel_ms'], 3),
                       'roofline_frac': round(rr['achieved_gbs'] / PEAK_HBM_GBS, 4),
                       'achieved_gbs': round(rr['achieved_gbs'], 1), 'bytes_per_cell': BYTES_PER_CELL[form],
                       'traffic': load_traffic(form)}
@@ -209,7 +233,7 @@ def main():
                            'queue_delay_s_p50': round(serving['queue_delay_s']['p50'], 3),
                            'queue_delay_s_p99': round(serving['queue_delay_s']['p99'], 3),
                            'mlups': round(serving['mlups'], 1), 'window_s': args.serve_seconds,
-                           'sessions_admitted': len(serving.get('admitted', [])) or None,
+                           'sessions_admitted': serving['sessions_admitted'],
                            'sessions_done': serving['sessions_done'],
                            'workload': 'Poisson 2 sessions/s, 0.5M-33M cells log-uniform, 200-5000 steps, '
                                        'half fp32 half fp16, forces every step, slice every 20 steps'}

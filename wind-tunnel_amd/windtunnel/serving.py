@@ -287,7 +287,8 @@ class Server:
         lat = np.array(self.latencies) if self.latencies else np.zeros(1)
         qd = np.array(self.queue_delays) if self.queue_delays else np.zeros(1)
         return {
-            'wall_s': wall, 'sessions_done': len(self.finished), 'ticks': self.ticks,
+            'wall_s': wall, 'sessions_admitted': len(self.queue_delays), 'sessions_done': len(self.finished),
+            'ticks': self.ticks,
             'cell_steps': self.cell_steps, 'mlups': self.cell_steps / max(wall, 1e-9) / 1e6,
             'step_latency_ms': {'p50': float(np.percentile(lat, 50) * 1e3), 'p99': float(np.percentile(lat, 99) * 1e3),
                                 'max': float(lat.max() * 1e3)},

@@ -141,10 +141,21 @@ class Solver:
         if steps > 0:
             self.T.step(steps, p['uIn'], p['tau'], p['smag'], flags=_lib.WT_FLAG_FORCES)
             self._forces_fresh = True
+        self.advance_particles(steps, p)
+
+    def advance_particles(self, steps, p):
+        """The particle half of step(): one advection with dt = steps (lbm.py:448-453)."""
+        steps = int(steps)
         if self.num_p and steps:
             self.seed += 1
             self.T.advect(self.pos, self.spd, float(steps), p['uIn'], p['maxAge'], self.seed,
                           (p['seedX'], p['seedCy'], p['seedCz'], p['seedHy'], p['seedHz']))
+
+    def set_forces(self, sums):
+        """Take the force of the current state from a batched step's FP64 sums
+        (device tensor of 4: fx, fy, fz, mass) instead of recomputing it."""
+        self.T.sums.copy_(sums)
+        self._forces_fresh = True
 
     # lbm.py:455-456
     def slice_dims(self, axis):

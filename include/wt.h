@@ -33,8 +33,11 @@ extern "C" {
 #define WT_FLAG_MASS 2   /* fuse the FP64 mass reduction into the last step */
 
 /* One tunnel's device state. n = nx*ny*nz; sizes come from wt_buffer_sizes().
- * f[0], f[1]: ping-pong population planes, SoA: population i of cell
- *             idx = x + nx*(y + ny*z) at element i*n + idx (lbm.py:68, 118).
+ * f[0], f[1]: ping-pong populations, row-blocked SoA: each x-row
+ *             r = y + ny*z holds its 19 planes of nx values in turn, so
+ *             population i of cell idx = x + nx*r is element
+ *             19*nx*r + nx*i + x (the reference's plane i*n + idx,
+ *             lbm.py:68, 118, cut into rows). 19*n elements.
  * cls:        per-cell class byte built by wt_set_solid from the solid mask.
  * mac:        float4 (ux, uy, uz, rho) per cell (lbm.py:406).
  * shell[0,1]: ping-pong snapshot of the macroscopic field on the six layers
@@ -132,9 +135,11 @@ int wt_batch_prepare(const wt_grid* grids, const wt_params* params, const int* c
 int wt_batch_step(void* workspace, const wt_batch_info* info, int parity, int flags, void* stream);
 
 /* Launch shape of the step kernels of one storage form: cells per thread
- * (1, 2 or 4) and tile order flags: bit 0 XCD-contiguous tiles, bit 1 sweep
- * direction alternating with the buffer parity. Chosen by the autotuner;
- * results do not depend on it. Applies to later wt_step / wt_batch_prepare. */
+ * (1, 2 or 4) and flags: bit 0 XCD-contiguous tiles, bit 1 sweep direction
+ * alternating with the buffer parity, bit 2 (wt_step only) 64-bit global
+ * population addresses even where 32-bit buffer offsets would do (the form
+ * large tunnels take). Chosen by the autotuner; results do not depend on it.
+ * Applies to later wt_step / wt_batch_prepare. */
 int wt_set_launch_shape(int form, int cpt, int remap);
 int wt_get_launch_shape(int form, int* cpt, int* remap);
 

@@ -130,7 +130,9 @@ def test_fused_force_equals_standalone(gpu):
     assert _raw_equal(fused[:3], alone[:3])
 
 
-@pytest.mark.parametrize('cpt,remap', [(1, 0), (2, 1), (4, 1), (4, 0), (1, 2), (2, 3)])
+# remap bit 2 switches the fast path from 32-bit buffer offsets to 64-bit global
+# addresses (the form tunnels beyond 32-bit byte offsets take)
+@pytest.mark.parametrize('cpt,remap', [(1, 0), (2, 1), (4, 1), (4, 0), (1, 2), (2, 3), (1, 4), (2, 5)])
 def test_launch_shape_does_not_change_results(gpu, cpt, remap):
     from windtunnel import tunnel as T
     from windtunnel._lib import WT_FLAG_FORCES, WT_FLAG_MASS
@@ -142,6 +144,21 @@ def test_launch_shape_does_not_change_results(gpu, cpt, remap):
         t.shape = shape
         t.step(11, 0.05, 0.52, 0.17, flags=WT_FLAG_FORCES | WT_FLAG_MASS)
         assert T.get_launch_shape('fp32') == shape
+        res.append((t.raw_populations().cpu().numpy(), t.mac.cpu().numpy(), t.sums.cpu().numpy()))
+    for a, b in zip(res[0], res[1]):
+        assert _raw_equal(a, b)
+
+
+@pytest.mark.parametrize('form', ['fp16', 'bf16'])
+def test_global_addressing_matches_buffer_addressing(gpu, form):
+    from windtunnel._lib import WT_FLAG_FORCES
+    dims = (70, 26, 22)
+    mask = _car_mask(dims, seed=9)
+    res = []
+    for shape in ((1, 0), (1, 4)):
+        t = _tunnel(dims, form, mask, 0.05)
+        t.shape = shape
+        t.step(9, 0.05, 0.52, 0.17, flags=WT_FLAG_FORCES)
         res.append((t.raw_populations().cpu().numpy(), t.mac.cpu().numpy(), t.sums.cpu().numpy()))
     for a, b in zip(res[0], res[1]):
         assert _raw_equal(a, b)

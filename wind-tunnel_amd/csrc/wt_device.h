@@ -31,6 +31,22 @@ constexpr int opp(int i) {
   return t[i];
 }
 // lattice weight in double, lbm.py:18
+// Population layout: row-blocked SoA. Each x-row r = y + ny*z of nx cells
+// stores its 19 planes of nx values one after another: population i of cell
+// idx = x + nx*r is element 19*nx*r + nx*i + x. A wave's consecutive cells read
+// or write one contiguous run per plane, and a row's 19 runs lie together.
+// Interior cells never pull across their own row in x and y/z neighbours are
+// whole rows away, so population i of the upstream neighbour idx - c_i sits at
+// the fixed offset pshift(i) from the cell's own plane-0 element prow(idx, r).
+__host__ __device__ __forceinline__ uint32_t prow(uint32_t idx, uint32_t row, uint32_t nx) {
+  return idx + (uint32_t)(Q - 1) * nx * row;
+}
+// pshift in 32-bit wrap-around arithmetic: prow + pshift is the exact element
+// index (< 19 n < 2^32) even where the signed offset is negative.
+__host__ __device__ __forceinline__ uint32_t pshift(int i, uint32_t nx, uint32_t nxy) {
+  return (uint32_t)i * nx - ((uint32_t)cx(i) + (uint32_t)(Q * cy(i)) * nx + (uint32_t)(Q * cz(i)) * nxy);
+}
+
 constexpr double wd(int i) { return i == 0 ? 1.0 / 3.0 : (i <= 6 ? 1.0 / 18.0 : 1.0 / 36.0); }
 
 // class byte written by wt_set_solid
@@ -117,6 +133,7 @@ struct StepDesc {
   uint32_t tile_begin; // first global tile of this tunnel (batched launches)
   uint32_t ntiles;
   int32_t flags;
+  int32_t global_addr; // 1: populations beyond 32-bit buffer offsets, use 64-bit global addresses
 };
 
 // Shell layout: six layers next to the faces, 4 reals per entry.

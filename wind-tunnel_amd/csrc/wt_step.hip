@@ -11,9 +11,11 @@
 // mass sum of the cells it just updated, reduced in FP64 with wavefront
 // shuffles into fixed slots so the result does not depend on scheduling.
 //
-// Layout: populations are SoA planes (population i of cell idx at i*n + idx).
-// A workgroup is 256 threads = 4 wave64; lanes run along x, so each population
-// load or store of a wave is one contiguous 256 B (FP32) or 128 B (FP16) run.
+// Layout: populations are row-blocked SoA (wt_device.h: each x-row holds its 19
+// planes in turn). A workgroup is 256 threads = 4 wave64; lanes run along x, so
+// each population load or store of a wave is one contiguous 256 B (FP32) or
+// 128 B (FP16) run, and the 19 stores of a row land in one region (38 KiB for
+// an FP32 row of 512 cells) instead of 19 planes n cells apart.
 // Link classes come from the class byte built once per geometry: only cells
 // flagged as touching a solid gather their 18 neighbour classes.
 //
@@ -295,13 +297,12 @@ __device__ __forceinline__ void step_cell_general(const StepDesc& d, const typen
                                                   typename Form<FORM>::elem* fout,
                                                   const Shell<typename Form<FORM>::real>& shin,
                                                   const Shell<typename Form<FORM>::real>& shout,
-                                                  uint32_t idx, uint8_t c, int x, int y, int z, bool face,
-                                                  int flags, Acc& acc) {
+                                                  uint32_t idx, uint32_t pc, uint8_t c, int x, int y, int z,
+                                                  bool face, int flags, Acc& acc) {
   typedef Form<FORM> S;
   typedef typename S::real R;
   typedef typename S::elem E;
-  const int32_t nx = d.nx, ny = d.ny, nz = d.nz;
-  const uint32_t n = d.n;
+  const int32_t nx = d.nx, ny = d.ny;
   const int32_t nxy = nx * ny;
   const R zero = (R)0, one = (R)1;
 
@@ -316,7 +317,7 @@ __device__ __forceinline__ void step_cell_general(const StepDesc& d, const typen
     const R bsq = (R)1.5 * (bux * bux + buy * buy + buz * buz);
 #pragma unroll
     for (int i = 0; i < Q; i++)
-      st_stream(fout + (size_t)i * n + idx, S::store(face_feq<R>(i, br, bux, buy, buz, bsq), i));
+      st_stream(fout + (pc + (uint32_t)i * (uint32_t)nx), S::store(face_feq<R>(i, br, bux, buy, buz, bsq), i));
     d.mac[idx] = make_float4((float)bux, (float)buy, (float)buz, (float)br);
     if (flags & WT_FLAG_MASS) acc.m += (double)br;
     return;
@@ -329,10 +330,10 @@ __device__ __forceinline__ void step_cell_general(const StepDesc& d, const typen
 #pragma unroll
   for (int i = 0; i < Q; i++) {
     if (i == 0) {
-      f[0] = S::load(ld_stream(fin + idx), 0);
+      f[0] = S::load(ld_stream(fin + pc), 0);
     } else {
       const bool b = (lk.bb >> i) & 1u;
-      const size_t a = b ? (size_t)opp(i) * n + idx : (size_t)i * n + (idx - offs(i, nx, nxy));
+      const uint32_t a = pc + (b ? (uint32_t)opp(i) * (uint32_t)nx : pshift(i, nx, nxy));
       f[i] = S::load(ld_stream(fin + a), i);
     }
   }
@@ -348,7 +349,7 @@ __device__ __forceinline__ void step_cell_general(const StepDesc& d, const typen
 #pragma unroll
   for (int i = 0; i < Q; i++) {
     st[i] = S::store(out[i], i);
-    st_stream(fout + (size_t)i * n + idx, st[i]);
+    st_stream(fout + (pc + (uint32_t)i * (uint32_t)nx), st[i]);
   }
   d.mac[idx] = make_float4((float)ux, (float)uy, (float)uz, (float)rho);
   shell_write<R>(shout, x, y, z, ux, uy, uz, rho);
@@ -357,11 +358,13 @@ __device__ __forceinline__ void step_cell_general(const StepDesc& d, const typen
 }
 
 // ---- one cell, fast path: the whole wave is interior fluid away from walls -----------
-// Every lane pulls from its 18 neighbours, so population i of the wave is one
-// contiguous run at a per-plane offset held in a scalar register plus the
-// lane's byte offset. For 2- and 4-byte storage the planes are addressed as raw
-// buffers (one resource for all 19 planes, hardware range check on), so a load
-// or store costs no vector address arithmetic at all.
+// Every lane pulls from its 18 neighbours. With the row-blocked layout the load
+// of population i is the lane's own element offset (one vector register shared
+// by all 38 loads and stores) plus the wave-uniform pshift(i) in a scalar
+// register. For 2- and 4-byte storage the populations are addressed as one raw
+// buffer (hardware range check on) whose base sits `bias` bytes below the
+// allocation so that every scalar offset is non-negative; buffers beyond 32-bit
+// byte offsets use 64-bit global addresses instead.
 constexpr int BUF_GFX9_WORD3 = 0x00020000;  // raw buffer, gfx9-family resource word 3
 constexpr int CPOL_LD = WT_LOAD_NT ? 2 : 0;   // non-temporal cache policy bit
 constexpr int CPOL_ST = WT_STORE_NT ? 2 : 0;
@@ -389,14 +392,15 @@ __device__ __forceinline__ void step_cell_fast(const StepDesc& d, const typename
                                                typename Form<FORM>::elem* fout,
                                                const Shell<typename Form<FORM>::real>& shin,
                                                const Shell<typename Form<FORM>::real>& shout, uint32_t idx,
-                                               int x, int y, int z, bool xface, int flags, Acc& acc) {
+                                               uint32_t pc, int x, int y, int z, bool xface, int flags,
+                                               Acc& acc) {
   typedef Form<FORM> S;
   typedef typename S::real R;
   typedef typename S::elem E;
-  const int32_t nx = d.nx;
-  const uint32_t n = d.n;
-  const int32_t nxy = nx * d.ny;
-  const uint32_t bo = idx * (uint32_t)sizeof(E);
+  const uint32_t nx = (uint32_t)d.nx, n = d.n, nxy = nx * (uint32_t)d.ny;
+  constexpr uint32_t ES = (uint32_t)sizeof(E);
+  const uint32_t bo = pc * ES;
+  const uint32_t bias = (uint32_t)Q * (nxy + nx) * ES;
   R out[Q], ux, uy, uz, rho;
   if (xface) {
     // inlet / outlet lanes of this wave (lbm.py:84-109)
@@ -406,31 +410,39 @@ __device__ __forceinline__ void step_cell_fast(const StepDesc& d, const typename
     for (int i = 0; i < Q; i++) out[i] = face_feq<R>(i, rho, ux, uy, uz, bsq);
   } else {
     R f[Q];
-    if constexpr (sizeof(E) <= 4) {
-      const int bytes = (int)(19u * n * (uint32_t)sizeof(E));
-      const __amdgpu_buffer_rsrc_t rin = __builtin_amdgcn_make_buffer_rsrc((void*)fin, (short)0, bytes, BUF_GFX9_WORD3);
+    auto global_loads = [&]() {
 #pragma unroll
-      for (int i = 0; i < Q; i++) {
-        const uint32_t so = ((uint32_t)i * n - (uint32_t)offs(i, nx, nxy)) * (uint32_t)sizeof(E);
-        f[i] = S::load(RawIO<E>::ld(rin, bo, so), i);
+      for (int i = 0; i < Q; i++) f[i] = S::load(ld_stream(fin + (pc + pshift(i, nx, nxy))), i);
+    };
+    if constexpr (sizeof(E) <= 4) {
+      if (!d.global_addr) {
+        const __amdgpu_buffer_rsrc_t rin = __builtin_amdgcn_make_buffer_rsrc(
+            (void*)((const char*)fin - bias), (short)0, (int)((uint32_t)Q * n * ES + bias), BUF_GFX9_WORD3);
+#pragma unroll
+        for (int i = 0; i < Q; i++) f[i] = S::load(RawIO<E>::ld(rin, bo, pshift(i, nx, nxy) * ES + bias), i);
+      } else {
+        global_loads();
       }
     } else {
-#pragma unroll
-      for (int i = 0; i < Q; i++) {
-        const char* plane = (const char*)(fin + (size_t)i * n - (ptrdiff_t)offs(i, nx, nxy));
-        f[i] = S::load(ld_stream((const E*)(plane + bo)), i);
-      }
+      global_loads();
     }
     collide<R>(f, x, nx, (R)d.tau, (R)d.smag, out, ux, uy, uz, rho);
   }
+  auto global_stores = [&]() {
+#pragma unroll
+    for (int i = 0; i < Q; i++) st_stream(fout + (pc + (uint32_t)i * nx), S::store(out[i], i));
+  };
   if constexpr (sizeof(E) <= 4) {
-    const int bytes = (int)(19u * n * (uint32_t)sizeof(E));
-    const __amdgpu_buffer_rsrc_t rout = __builtin_amdgcn_make_buffer_rsrc((void*)fout, (short)0, bytes, BUF_GFX9_WORD3);
+    if (!d.global_addr) {
+      const __amdgpu_buffer_rsrc_t rout =
+          __builtin_amdgcn_make_buffer_rsrc((void*)fout, (short)0, (int)((uint32_t)Q * n * ES), BUF_GFX9_WORD3);
 #pragma unroll
-    for (int i = 0; i < Q; i++) RawIO<E>::st(rout, bo, (uint32_t)i * n * (uint32_t)sizeof(E), S::store(out[i], i));
+      for (int i = 0; i < Q; i++) RawIO<E>::st(rout, bo, (uint32_t)i * nx * ES, S::store(out[i], i));
+    } else {
+      global_stores();
+    }
   } else {
-#pragma unroll
-    for (int i = 0; i < Q; i++) st_stream((E*)((char*)(fout + (size_t)i * n) + bo), S::store(out[i], i));
+    global_stores();
   }
   d.mac[idx] = make_float4((float)ux, (float)uy, (float)uz, (float)rho);
   if (!xface) shell_write<R>(shout, x, y, z, ux, uy, uz, rho);
@@ -449,15 +461,16 @@ __device__ __forceinline__ void step_cell(const StepDesc& d, const typename Form
   const uint32_t zz = divu(yz, d.my, d.sy);
   const int y = (int)(yz - zz * (uint32_t)ny);
   const int z = (int)zz;
+  const uint32_t pc = prow(idx, yz, (uint32_t)nx);
   const uint8_t c = d.cls[idx];
   const bool xface = (x == 0) | (x == nx - 1);
   const bool face = xface | (y == 0) | (y == ny - 1) | (z == 0) | (z == nz - 1);
   // fluid lanes away from walls, and fluid inlet/outlet lanes, take the fast path
   const bool simple = (c == 0) & (!face | xface) & (y != 0) & (y != ny - 1) & (z != 0) & (z != nz - 1);
   if (__all(simple))
-    step_cell_fast<FORM>(d, fin, fout, shin, shout, idx, x, y, z, xface, flags, acc);
+    step_cell_fast<FORM>(d, fin, fout, shin, shout, idx, pc, x, y, z, xface, flags, acc);
   else
-    step_cell_general<FORM>(d, fin, fout, shin, shout, idx, c, x, y, z, face, flags, acc);
+    step_cell_general<FORM>(d, fin, fout, shin, shout, idx, pc, c, x, y, z, face, flags, acc);
 }
 
 // ---- deterministic FP64 reduction ------------------------------------------------
@@ -653,11 +666,12 @@ __global__ __launch_bounds__(BLOCK) void k_force(StepDesc d) {
     if (c & CLS_FORCE) {
       const E* f = (const E*)d.fin;
       const int32_t nx = d.nx, nxy = d.nx * d.ny;
+      const uint32_t pc = prow(idx, divu(idx, d.mx, d.sx), (uint32_t)nx);
       const uint32_t fl = gather_force_links(d.cls, idx, nx, nxy);
 #pragma unroll
       for (int i = 1; i < Q; i++) {
         if ((fl >> i) & 1u) {
-          const R v = (R)2.0 * (S::load(f[(size_t)i * d.n + idx], i) - (R)wd(i));
+          const R v = (R)2.0 * (S::load(f[pc + (uint32_t)i * (uint32_t)nx], i) - (R)wd(i));
           const double dv = (double)v;
           if (cx(i) > 0) a.fx += dv; else if (cx(i) < 0) a.fx -= dv;
           if (cy(i) > 0) a.fy += dv; else if (cy(i) < 0) a.fy -= dv;
@@ -674,19 +688,20 @@ __global__ __launch_bounds__(BLOCK) void k_force(StepDesc d) {
 template <int FORM>
 __global__ __launch_bounds__(BLOCK) void k_init(typename Form<FORM>::elem* f0,
                                                 typename Form<FORM>::elem* f1, float4* mac,
-                                                uint32_t n, float uin_f) {
+                                                uint32_t n, uint32_t nx, float uin_f) {
   typedef Form<FORM> S;
   typedef typename S::real R;
   const uint32_t idx = blockIdx.x * BLOCK + threadIdx.x;
   if (idx >= n) return;
+  const uint32_t pc = prow(idx, idx / nx, nx);
   const R uin = (R)uin_f;
 #pragma unroll
   for (int i = 0; i < Q; i++) {
     const R v = (R)wd(i) * ((R)1.0 + (R)(3 * cx(i)) * uin + (R)(4.5 * cx(i) * cx(i)) * uin * uin -
                             (R)1.5 * uin * uin);
     const typename S::elem e = S::store(v, i);
-    f0[(size_t)i * n + idx] = e;
-    f1[(size_t)i * n + idx] = e;
+    f0[pc + (uint32_t)i * nx] = e;
+    f1[pc + (uint32_t)i * nx] = e;
   }
   mac[idx] = make_float4(uin_f, 0.f, 0.f, 1.f);
 }
@@ -780,6 +795,10 @@ StepDesc make_desc(const wt_grid* g, int cur, const wt_params* p, int flags) {
   d.tile_begin = 0;
   d.ntiles = 0;
   d.flags = flags;
+  // raw-buffer addressing needs the population bytes plus the scalar-offset bias in 32 bits
+  const int64_t eb = form_elem_bytes(g->form);
+  const int64_t span = (int64_t)Q * d.n * eb + (int64_t)Q * ((int64_t)g->nx * g->ny + g->nx) * eb;
+  d.global_addr = (eb > 4 || span >= ((int64_t)1 << 32)) ? 1 : 0;
   return d;
 }
 
@@ -789,7 +808,7 @@ int check_grid(const wt_grid* g, bool need_state) {
     return fail(-1, "grid dimensions must be at least 3 (got %d x %d x %d)", g->nx, g->ny, g->nz);
   const int64_t n = (int64_t)g->nx * g->ny * g->nz;
   if (n >= (int64_t)1 << 31) return fail(-1, "grid of %lld cells exceeds 2^31 - 1", (long long)n);
-  if ((int64_t)19 * n >= ((int64_t)1 << 32))
+  if ((int64_t)Q * n >= ((int64_t)1 << 32))
     return fail(-1, "grid of %lld cells: 19*n population elements exceed 2^32", (long long)n);
   if (g->form < WT_FP32 || g->form > WT_FP64) return fail(-1, "unknown storage form %d", g->form);
   if (need_state && (!g->f[0] || !g->f[1] || !g->cls || !g->mac || !g->shell[0] || !g->shell[1]))
@@ -862,7 +881,7 @@ extern "C" {
 int wt_set_launch_shape(int form, int cpt, int remap) {
   if (form < WT_FP32 || form > WT_FP64) return fail(-1, "unknown storage form %d", form);
   if (cpt != 1 && cpt != 2 && cpt != 4) return fail(-1, "cells per thread must be 1, 2 or 4 (got %d)", cpt);
-  g_shape[form] = Shape{cpt, remap & 3};
+  g_shape[form] = Shape{cpt, remap & 7};
   return 0;
 }
 
@@ -877,7 +896,7 @@ int wt_buffer_sizes(int nx, int ny, int nz, int form, int64_t sizes[6]) {
   if (nx < 3 || ny < 3 || nz < 3) return fail(-1, "grid dimensions must be at least 3");
   if (form < WT_FP32 || form > WT_FP64) return fail(-1, "unknown storage form %d", form);
   const int64_t n = (int64_t)nx * ny * nz;
-  sizes[0] = 19 * n * form_elem_bytes(form);
+  sizes[0] = (int64_t)Q * n * form_elem_bytes(form);
   sizes[1] = n;
   sizes[2] = 16 * n;
   sizes[3] = 4 * shell_entries(nx, ny, nz) * form_real_bytes(form);
@@ -904,16 +923,16 @@ int wt_init(const wt_grid* g, float uin, void* stream) {
   const dim3 sgrid((ent + BLOCK - 1) / BLOCK);
   switch (g->form) {
     case WT_FP32:
-      hipLaunchKernelGGL(k_init<WT_FP32>, grid, dim3(BLOCK), 0, s, (float*)g->f[0], (float*)g->f[1], mac, n, uin);
+      hipLaunchKernelGGL(k_init<WT_FP32>, grid, dim3(BLOCK), 0, s, (float*)g->f[0], (float*)g->f[1], mac, n, (uint32_t)g->nx, uin);
       break;
     case WT_FP16:
-      hipLaunchKernelGGL(k_init<WT_FP16>, grid, dim3(BLOCK), 0, s, (uint16_t*)g->f[0], (uint16_t*)g->f[1], mac, n, uin);
+      hipLaunchKernelGGL(k_init<WT_FP16>, grid, dim3(BLOCK), 0, s, (uint16_t*)g->f[0], (uint16_t*)g->f[1], mac, n, (uint32_t)g->nx, uin);
       break;
     case WT_BF16:
-      hipLaunchKernelGGL(k_init<WT_BF16>, grid, dim3(BLOCK), 0, s, (uint16_t*)g->f[0], (uint16_t*)g->f[1], mac, n, uin);
+      hipLaunchKernelGGL(k_init<WT_BF16>, grid, dim3(BLOCK), 0, s, (uint16_t*)g->f[0], (uint16_t*)g->f[1], mac, n, (uint32_t)g->nx, uin);
       break;
     case WT_FP64:
-      hipLaunchKernelGGL(k_init<WT_FP64>, grid, dim3(BLOCK), 0, s, (double*)g->f[0], (double*)g->f[1], mac, n, uin);
+      hipLaunchKernelGGL(k_init<WT_FP64>, grid, dim3(BLOCK), 0, s, (double*)g->f[0], (double*)g->f[1], mac, n, (uint32_t)g->nx, uin);
       break;
   }
   if (g->form == WT_FP64)
@@ -941,7 +960,8 @@ int wt_step(const wt_grid* g, int cur, int nsteps, const wt_params* p, int flags
     return fail(-1, "reduction flags need partials and sums buffers");
   hipStream_t s = (hipStream_t)stream;
   const Shape sh = g_shape[g->form];
-  const StepDesc d = make_desc(g, cur, p, 0);
+  StepDesc d = make_desc(g, cur, p, 0);
+  if (sh.remap & 4) d.global_addr = 1;  // shape bit 2: 64-bit global addressing at any size
   for (int k = 0; k < nsteps; k++) {
     const int f = (k == nsteps - 1) ? (flags & (WT_FLAG_FORCES | WT_FLAG_MASS)) : 0;
     // with shape bit 1 the sweep direction alternates with the buffer parity

@@ -1,8 +1,8 @@
 """Device state of one wind tunnel and thin wrappers over the libwt kernels.
 
 torch provides the device buffers and the stream; every computation is a
-libwt call. Buffer layout (see include/wt.h): two SoA population buffers
-f[0], f[1] of 19*n elements of the storage form, a class byte per cell, the
+libwt call. Buffer layout (see include/wt.h): two row-blocked SoA
+population buffers f[0], f[1] (each x-row holds its 19 planes) of 19*n elements, a class byte per cell, the
 float4 macroscopic field mac (ux, uy, uz, rho), two face-layer snapshots, and
 the FP64 reduction slots.
 """
@@ -108,9 +108,13 @@ class Tunnel:
 
     # ---- views ---------------------------------------------------------------
     def raw_populations(self, which=None):
-        """f[which] (default f[cur]) as its storage dtype, shape (19, n), on the device."""
+        """f[which] (default f[cur]) as its storage dtype, shape (19, n) in the
+        reference's plane order (population i of cell idx at [i, idx]), on the
+        device. The buffer itself is row-blocked SoA (each x-row holds its 19
+        planes in turn); this returns a reordered copy."""
         k = self.cur if which is None else which
-        return self.f[k].view(ELEM_DTYPE[self.form]).view(19, self.n)
+        rows = self.f[k].view(ELEM_DTYPE[self.form]).view(self.ny * self.nz, 19, self.nx)
+        return rows.permute(1, 0, 2).reshape(19, self.n)
 
     def populations(self, which=None, dtype=np.float64):
         """f_i of every cell on the host, shape (19, n), reconstructed from the storage form."""

@@ -155,6 +155,19 @@ int wt_advect(const wt_grid* g, float* pos, float* spd, int64_t num_p, float dt,
 int wt_advect_ordered(const wt_grid* g, float* pos, float* spd, const uint32_t* order, int64_t num_p,
                       float dt, float uin, float max_age, uint32_t seed, const wt_seed_box* box,
                       void* stream);
+/* Slot storage: slot j of pos/spd holds particle ids[j] (a permutation of
+ * 0..num_p-1), e.g. after wt_particle_gather put the particles in cell order,
+ * so a wave reads and writes contiguous slots. The random stream is keyed by
+ * the id, so every particle's result equals wt_advect's bit for bit. No
+ * counterpart in the reference (advect_kernel, lbm.py:233-263, in another
+ * storage order). */
+int wt_advect_slots(const wt_grid* g, float* pos, float* spd, const uint32_t* ids, int64_t num_p, float dt,
+                    float uin, float max_age, uint32_t seed, const wt_seed_box* box, void* stream);
+/* pos_out[j] = pos[order[j]], spd_out[j] = spd[order[j]], ids_out[j] =
+ * ids[order[j]] (order[j] if ids is NULL); not in place. Puts slot storage in
+ * the order wt_particle_order returns. No counterpart in the reference. */
+int wt_particle_gather(const float* pos, const float* spd, const uint32_t* ids, const uint32_t* order,
+                       int64_t num_p, float* pos_out, float* spd_out, uint32_t* ids_out, void* stream);
 /* Device workspace bytes for wt_particle_order. */
 int64_t wt_particle_order_workspace_bytes(int64_t num_p);
 /* order[] = particle ids sorted by the index of the cell they are in (radix
@@ -164,6 +177,10 @@ int wt_particle_order(const wt_grid* g, const float* pos, int64_t num_p, uint32_
 /* Quantize particles (pack_particles, lbm.py:266-275). */
 int wt_pack_particles(const wt_grid* g, const float* pos, const float* spd, uint16_t* q16,
                       uint8_t* q8, int64_t num_p, void* stream);
+/* Same from slot storage: slot j goes to position ids[j] of q16/q8, so the
+ * output is in particle-id order (pack_particles, lbm.py:266-275). */
+int wt_pack_particles_slots(const wt_grid* g, const float* pos, const float* spd, const uint32_t* ids,
+                            uint16_t* q16, uint8_t* q8, int64_t num_p, void* stream);
 /* One axis-aligned slice as (value byte, fluid byte) pairs (slice_kernel,
  * lbm.py:278-318; Solver.frame lbm.py:464-474). index must be in range. */
 int wt_slice(const wt_grid* g, uint8_t* out, int axis, int index, int field, float uin, float lref,

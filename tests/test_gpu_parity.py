@@ -235,6 +235,45 @@ def test_ordered_advect_equals_plain(gpu):
     assert _raw_equal(a_spd.cpu().numpy(), b_spd.cpu().numpy())
 
 
+def test_slot_storage_advect_equals_plain(gpu):
+    """Particles stored in cell order (re-gathered twice) with an id array give
+    every particle the state plain id-order advection gives, and pack to the
+    same id-ordered bytes."""
+    t, mask = _flow_state()
+    rng = np.random.default_rng(11)
+    N = 20011
+    pos = np.column_stack([rng.uniform(-2, t.nx + 2, N), rng.uniform(-2, t.ny + 2, N), rng.uniform(-2, t.nz + 2, N),
+                           rng.uniform(0, 90, N)]).astype(np.float32)
+    pos[7] = np.nan
+    spd = np.ones(N, np.float32)
+    box = (2.0, 10.0, 11.0, 6.0, 5.0)
+    a_pos, a_spd = torch.from_numpy(pos.copy()).cuda(), torch.from_numpy(spd.copy()).cuda()
+    s_pos, s_spd = torch.from_numpy(pos.copy()).cuda(), torch.from_numpy(spd.copy()).cuda()
+    s_ids = torch.arange(N, dtype=torch.int32, device='cuda')
+    order = torch.empty(N, dtype=torch.int32, device='cuda')
+    ws = None
+    for k in range(6):
+        t.advect(a_pos, a_spd, 2.0, 0.05, 60.0, 80 + k, box)
+        if k in (0, 3):
+            ws = t.particle_order(s_pos, order, ws)
+            p2, v2, i2 = torch.empty_like(s_pos), torch.empty_like(s_spd), torch.empty_like(s_ids)
+            t.particle_gather(s_pos, s_spd, s_ids, order, p2, v2, i2)
+            s_pos, s_spd, s_ids = p2, v2, i2
+        t.advect_slots(s_pos, s_spd, s_ids, 2.0, 0.05, 60.0, 80 + k, box)
+    ids = s_ids.cpu().numpy().astype(np.int64)
+    assert np.array_equal(np.sort(ids), np.arange(N))
+    got_pos, got_spd = np.empty_like(pos), np.empty_like(spd)
+    got_pos[ids] = s_pos.cpu().numpy()
+    got_spd[ids] = s_spd.cpu().numpy()
+    assert _raw_equal(a_pos.cpu().numpy(), got_pos) and _raw_equal(a_spd.cpu().numpy(), got_spd)
+    qa16 = torch.zeros(4 * N, dtype=torch.int16, device='cuda')
+    qa8 = torch.zeros(N, dtype=torch.uint8, device='cuda')
+    qs16, qs8 = torch.zeros_like(qa16), torch.zeros_like(qa8)
+    t.pack_particles(a_pos, a_spd, qa16, qa8)
+    t.pack_particles_slots(s_pos, s_spd, s_ids, qs16, qs8)
+    assert torch.equal(qa16, qs16) and torch.equal(qa8, qs8)
+
+
 def test_pack_bit_exact(gpu):
     t, _ = _flow_state(steps=2)
     rng = np.random.default_rng(5)

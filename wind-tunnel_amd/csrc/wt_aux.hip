@@ -80,7 +80,8 @@ __device__ __forceinline__ float rng_f(uint32_t& st) {
 struct AdvectArgs {
   const float4* mac;
   const uint8_t* cls;
-  const uint32_t* order;  // processing order (a permutation of particle ids) or null
+  const uint32_t* order;  // processing order (a permutation of particle slots) or null
+  const uint32_t* ids;    // particle id held in each slot (slot storage) or null (slot = id)
   float4* pos;
   float* spd;
   uint32_t num_p;
@@ -94,9 +95,11 @@ struct AdvectArgs {
 __global__ __launch_bounds__(ABLOCK) void k_advect(AdvectArgs a) {
   const uint32_t j = blockIdx.x * ABLOCK + threadIdx.x;
   if (j >= a.num_p) return;
-  // particle id: lanes of a wave take spatial neighbours when an order is given
-  const uint32_t i = a.order ? a.order[j] : j;
-  const float4 p = a.pos[i];
+  // memory slot: lanes of a wave take spatial neighbours when an order is given
+  const uint32_t slot = a.order ? a.order[j] : j;
+  // particle id, which keys the respawn stream: the slot itself unless slots hold other ids
+  const uint32_t i = a.ids ? a.ids[slot] : slot;
+  const float4 p = a.pos[slot];
   float qx = p.x, qy = p.y, qz = p.z;
   const float4 m1 = sample_mac(a.mac, qx, qy, qz, a.nx, a.ny, a.nz);
   const float h = 0.5f * a.dt;
@@ -122,8 +125,22 @@ __global__ __launch_bounds__(ABLOCK) void k_advect(AdvectArgs a) {
     age = 0.0f;
     s = 1.0f;
   }
-  a.pos[i] = make_float4(qx, qy, qz, age);
-  a.spd[i] = s;
+  a.pos[slot] = make_float4(qx, qy, qz, age);
+  a.spd[slot] = s;
+}
+
+// Slot storage: out[j] = in[order[j]] for the particle arrays and their ids.
+__global__ __launch_bounds__(ABLOCK) void k_gather(const float4* __restrict__ pos, const float* __restrict__ spd,
+                                                   const uint32_t* __restrict__ ids,
+                                                   const uint32_t* __restrict__ order, uint32_t num_p,
+                                                   float4* __restrict__ pos_out, float* __restrict__ spd_out,
+                                                   uint32_t* __restrict__ ids_out) {
+  const uint32_t j = blockIdx.x * ABLOCK + threadIdx.x;
+  if (j >= num_p) return;
+  const uint32_t k = order[j];
+  pos_out[j] = pos[k];
+  spd_out[j] = spd[k];
+  ids_out[j] = ids ? ids[k] : k;
 }
 
 // Sort key of each particle: the index of the cell its position falls in
@@ -141,18 +158,20 @@ __global__ __launch_bounds__(ABLOCK) void k_particle_keys(const float4* __restri
 
 // lbm.py:266-275
 __global__ __launch_bounds__(ABLOCK) void k_pack(const float4* __restrict__ pos, const float* __restrict__ spd,
-                                                 ushort4* q16, uint8_t* q8, uint32_t num_p, int nx, int ny,
-                                                 int nz) {
-  const uint32_t i = blockIdx.x * ABLOCK + threadIdx.x;
-  if (i >= num_p) return;
-  const float4 p = pos[i];
+                                                 const uint32_t* __restrict__ ids, ushort4* q16, uint8_t* q8,
+                                                 uint32_t num_p, int nx, int ny, int nz) {
+  const uint32_t j = blockIdx.x * ABLOCK + threadIdx.x;
+  if (j >= num_p) return;
+  // outputs are in particle-id order whatever slot holds the particle
+  const uint32_t i = ids ? ids[j] : j;
+  const float4 p = pos[j];
   ushort4 q;
   q.x = (uint16_t)(int)(clampf(p.x / (float)nx, 0.0f, 1.0f) * 65535.0f + 0.5f);
   q.y = (uint16_t)(int)(clampf(p.y / (float)ny, 0.0f, 1.0f) * 65535.0f + 0.5f);
   q.z = (uint16_t)(int)(clampf(p.z / (float)nz, 0.0f, 1.0f) * 65535.0f + 0.5f);
   q.w = (uint16_t)(int)clampf(p.w, 0.0f, 65535.0f);
   q16[i] = q;
-  q8[i] = (uint8_t)(int)(clampf(spd[i] / 2.0f, 0.0f, 1.0f) * 255.0f + 0.5f);
+  q8[i] = (uint8_t)(int)(clampf(spd[j] / 2.0f, 0.0f, 1.0f) * 255.0f + 0.5f);
 }
 
 struct SliceArgs {
@@ -305,8 +324,8 @@ int wt_advect(const wt_grid* g, float* pos, float* spd, int64_t num_p, float dt,
   if (num_p < 0 || num_p >= ((int64_t)1 << 31)) return fail(-1, "bad particle count %lld", (long long)num_p);
   if (num_p == 0) return 0;
   if (!pos || !spd || !box) return fail(-1, "null particle buffer or seed box");
-  AdvectArgs a{reinterpret_cast<const float4*>(g->mac), g->cls, nullptr, reinterpret_cast<float4*>(pos), spd,
-               (uint32_t)num_p, g->nx, g->ny, g->nz, dt, uin, max_age, seed, *box};
+  AdvectArgs a{reinterpret_cast<const float4*>(g->mac), g->cls, nullptr, nullptr, reinterpret_cast<float4*>(pos),
+               spd, (uint32_t)num_p, g->nx, g->ny, g->nz, dt, uin, max_age, seed, *box};
   hipLaunchKernelGGL(k_advect, dim3((unsigned)((num_p + ABLOCK - 1) / ABLOCK)), dim3(ABLOCK), 0,
                      (hipStream_t)stream, a);
   return check_launch("wt_advect");
@@ -318,11 +337,37 @@ int wt_advect_ordered(const wt_grid* g, float* pos, float* spd, const uint32_t* 
   if (num_p < 0 || num_p >= ((int64_t)1 << 31)) return fail(-1, "bad particle count %lld", (long long)num_p);
   if (num_p == 0) return 0;
   if (!pos || !spd || !box || !order) return fail(-1, "null particle buffer, order or seed box");
-  AdvectArgs a{reinterpret_cast<const float4*>(g->mac), g->cls, order, reinterpret_cast<float4*>(pos), spd,
-               (uint32_t)num_p, g->nx, g->ny, g->nz, dt, uin, max_age, seed, *box};
+  AdvectArgs a{reinterpret_cast<const float4*>(g->mac), g->cls, order, nullptr, reinterpret_cast<float4*>(pos),
+               spd, (uint32_t)num_p, g->nx, g->ny, g->nz, dt, uin, max_age, seed, *box};
   hipLaunchKernelGGL(k_advect, dim3((unsigned)((num_p + ABLOCK - 1) / ABLOCK)), dim3(ABLOCK), 0,
                      (hipStream_t)stream, a);
   return check_launch("wt_advect_ordered");
+}
+
+int wt_advect_slots(const wt_grid* g, float* pos, float* spd, const uint32_t* ids, int64_t num_p, float dt,
+                    float uin, float max_age, uint32_t seed, const wt_seed_box* box, void* stream) {
+  if (int e = check_small_grid(g)) return e;
+  if (num_p < 0 || num_p >= ((int64_t)1 << 31)) return fail(-1, "bad particle count %lld", (long long)num_p);
+  if (num_p == 0) return 0;
+  if (!pos || !spd || !box || !ids) return fail(-1, "null particle buffer, ids or seed box");
+  AdvectArgs a{reinterpret_cast<const float4*>(g->mac), g->cls, nullptr, ids, reinterpret_cast<float4*>(pos),
+               spd, (uint32_t)num_p, g->nx, g->ny, g->nz, dt, uin, max_age, seed, *box};
+  hipLaunchKernelGGL(k_advect, dim3((unsigned)((num_p + ABLOCK - 1) / ABLOCK)), dim3(ABLOCK), 0,
+                     (hipStream_t)stream, a);
+  return check_launch("wt_advect_slots");
+}
+
+int wt_particle_gather(const float* pos, const float* spd, const uint32_t* ids, const uint32_t* order,
+                       int64_t num_p, float* pos_out, float* spd_out, uint32_t* ids_out, void* stream) {
+  if (num_p < 0 || num_p >= ((int64_t)1 << 31)) return fail(-1, "bad particle count %lld", (long long)num_p);
+  if (num_p == 0) return 0;
+  if (!pos || !spd || !order || !pos_out || !spd_out || !ids_out) return fail(-1, "null particle buffer");
+  if (pos_out == pos || spd_out == spd || (ids && ids_out == ids))
+    return fail(-1, "wt_particle_gather cannot work in place");
+  hipLaunchKernelGGL(k_gather, dim3((unsigned)((num_p + ABLOCK - 1) / ABLOCK)), dim3(ABLOCK), 0,
+                     (hipStream_t)stream, reinterpret_cast<const float4*>(pos), spd, ids, order, (uint32_t)num_p,
+                     reinterpret_cast<float4*>(pos_out), spd_out, ids_out);
+  return check_launch("wt_particle_gather");
 }
 
 // Workspace: keys, sorted keys, ids (3 x num_p uint32, each 256-byte aligned), then radix-sort scratch.
@@ -376,9 +421,21 @@ int wt_pack_particles(const wt_grid* g, const float* pos, const float* spd, uint
   if (num_p == 0) return 0;
   if (!pos || !spd || !q16 || !q8) return fail(-1, "null particle buffer");
   hipLaunchKernelGGL(k_pack, dim3((unsigned)((num_p + ABLOCK - 1) / ABLOCK)), dim3(ABLOCK), 0,
-                     (hipStream_t)stream, reinterpret_cast<const float4*>(pos), spd,
+                     (hipStream_t)stream, reinterpret_cast<const float4*>(pos), spd, nullptr,
                      reinterpret_cast<ushort4*>(q16), q8, (uint32_t)num_p, g->nx, g->ny, g->nz);
   return check_launch("wt_pack_particles");
+}
+
+int wt_pack_particles_slots(const wt_grid* g, const float* pos, const float* spd, const uint32_t* ids,
+                            uint16_t* q16, uint8_t* q8, int64_t num_p, void* stream) {
+  if (!g) return fail(-1, "null grid");
+  if (num_p < 0 || num_p >= ((int64_t)1 << 31)) return fail(-1, "bad particle count %lld", (long long)num_p);
+  if (num_p == 0) return 0;
+  if (!pos || !spd || !ids || !q16 || !q8) return fail(-1, "null particle buffer or ids");
+  hipLaunchKernelGGL(k_pack, dim3((unsigned)((num_p + ABLOCK - 1) / ABLOCK)), dim3(ABLOCK), 0,
+                     (hipStream_t)stream, reinterpret_cast<const float4*>(pos), spd, ids,
+                     reinterpret_cast<ushort4*>(q16), q8, (uint32_t)num_p, g->nx, g->ny, g->nz);
+  return check_launch("wt_pack_particles_slots");
 }
 
 int wt_slice(const wt_grid* g, uint8_t* out, int axis, int index, int field, float uin, float lref, float vmin,

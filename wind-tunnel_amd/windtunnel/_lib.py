@@ -70,6 +70,9 @@ SIGNATURES = {
     'wt_particle_order_workspace_bytes': (c_i64, [c_i64]),
     'wt_particle_order': (c_i32, [P(Grid), c_vp, c_i64, c_vp, c_vp, c_i64, c_vp]),
     'wt_pack_particles': (c_i32, [P(Grid), c_vp, c_vp, c_vp, c_vp, c_i64, c_vp]),
+    'wt_advect_slots': (c_i32, [P(Grid), c_vp, c_vp, c_vp, c_i64, c_f32, c_f32, c_f32, c_u32, P(SeedBox), c_vp]),
+    'wt_particle_gather': (c_i32, [c_vp, c_vp, c_vp, c_vp, c_i64, c_vp, c_vp, c_vp, c_vp]),
+    'wt_pack_particles_slots': (c_i32, [P(Grid), c_vp, c_vp, c_vp, c_vp, c_vp, c_i64, c_vp]),
     'wt_slice': (c_i32, [P(Grid), c_vp, c_i32, c_i32, c_i32, c_f32, c_f32, c_f32, c_f32, c_vp]),
     'wt_cp': (c_i32, [P(Grid), c_vp, c_vp, c_vp, c_i64, c_f32, c_f32, c_vp]),
     'wt_link_classes': (c_i32, [P(Grid), c_vp, c_vp]),

@@ -53,9 +53,13 @@ class Session:
             d[:, 1] = cy + (rng.random(self.num_p) * 2 - 1) * hy
             d[:, 2] = cz + (rng.random(self.num_p) * 2 - 1) * hz
             d[:, 3] = rng.random(self.num_p) * self.max_age * 0.5
+            # slot storage: slot j holds particle ids[j]; every sort_every steps the
+            # slots are re-gathered in cell order so that advection reads and
+            # writes contiguous slots and samples neighbouring cells
             self.pos = torch.from_numpy(d).to(device)
             self.spd = torch.ones(self.num_p, dtype=torch.float32, device=device)
-            # particles are advected in cell order, refreshed every sort_every steps
+            self.ids = torch.arange(self.num_p, dtype=torch.int32, device=device)
+            self._spare = (torch.empty_like(self.pos), torch.empty_like(self.spd), torch.empty_like(self.ids))
             self.order = torch.empty(self.num_p, dtype=torch.int32, device=device)
             self.order_ws = None
             self.sort_every = 50
@@ -91,12 +95,24 @@ class Session:
         self.step_count += 1
 
     def advect(self, stream=None):
-        """Particle advection with dt = 1, in the spatial order refreshed every sort_every steps."""
+        """Particle advection with dt = 1 on slot storage re-sorted by cell every sort_every steps."""
         if self.step_count % self.sort_every == 0:
             self.order_ws = self.T.particle_order(self.pos, self.order, self.order_ws, stream=stream)
+            pos2, spd2, ids2 = self._spare
+            self.T.particle_gather(self.pos, self.spd, self.ids, self.order, pos2, spd2, ids2, stream=stream)
+            self._spare = (self.pos, self.spd, self.ids)
+            self.pos, self.spd, self.ids = pos2, spd2, ids2
         self.seed += 1
-        self.T.advect_ordered(self.pos, self.spd, self.order, 1.0, self.uin, self.max_age, self.seed, self.box,
-                              stream=stream)
+        self.T.advect_slots(self.pos, self.spd, self.ids, 1.0, self.uin, self.max_age, self.seed, self.box,
+                            stream=stream)
+
+    def particles_by_id(self):
+        """(pos, spd) on the host in particle-id order."""
+        ids = self.ids.cpu().numpy().astype(np.int64)
+        pos, spd = np.empty((self.num_p, 4), np.float32), np.empty(self.num_p, np.float32)
+        pos[ids] = self.pos.cpu().numpy()
+        spd[ids] = self.spd.cpu().numpy()
+        return pos, spd
 
     def emit_frame(self, stream=None):
         for (axis, index, field), dev, host in zip(self.slices, self.slice_dev, self.slice_host):

@@ -170,6 +170,23 @@ class Tunnel:
                                          float(uin), float(max_age), int(seed) & 0xffffffff, ctypes.byref(sb),
                                          stream_handle(stream)), 'wt_advect_ordered')
 
+    def advect_slots(self, pos, spd, ids, dt, uin, max_age, seed, box, stream=None):
+        """Advection of slot storage: slot j holds particle ids[j] (int32 tensor)."""
+        sb = SeedBox(*(float(v) for v in box))
+        check(self.lib.wt_advect_slots(self.gp, _ptr(pos), _ptr(spd), _ptr(ids), int(pos.shape[0]), float(dt),
+                                       float(uin), float(max_age), int(seed) & 0xffffffff, ctypes.byref(sb),
+                                       stream_handle(stream)), 'wt_advect_slots')
+
+    def particle_gather(self, pos, spd, ids, order, pos_out, spd_out, ids_out, stream=None):
+        """pos_out[j] = pos[order[j]] (and spd, ids; ids None = identity)."""
+        check(self.lib.wt_particle_gather(_ptr(pos), _ptr(spd), None if ids is None else _ptr(ids), _ptr(order),
+                                          int(pos.shape[0]), _ptr(pos_out), _ptr(spd_out), _ptr(ids_out),
+                                          stream_handle(stream)), 'wt_particle_gather')
+
+    def pack_particles_slots(self, pos, spd, ids, q16, q8, stream=None):
+        check(self.lib.wt_pack_particles_slots(self.gp, _ptr(pos), _ptr(spd), _ptr(ids), _ptr(q16), _ptr(q8),
+                                               int(pos.shape[0]), stream_handle(stream)), 'wt_pack_particles_slots')
+
     def particle_order(self, pos, order, workspace=None, stream=None):
         """order[:] = particle ids sorted by cell index; returns the workspace for reuse."""
         n = int(pos.shape[0])

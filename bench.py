@@ -65,14 +65,7 @@ def run_gpu(form, steps, warmup, mask, counts, lref):
     t_start.record(stream)
     for k in range(steps):
         is_force.append(s.is_frame())
-        ev[k][0].record(stream)
-        s.T.step(1, s.uin, s.tau, s.smag, flags=1 if s.is_frame() else 0)
-        ev[k][1].record(stream)
-        if s.num_p:
-            s.advect()
-        if s.is_frame():
-            s.emit_frame()
-        s.step_count += 1
+        s.step(stream=stream, events=ev[k])
     t_end.record(stream)
     torch.cuda.synchronize()
     wall = time.perf_counter() - h0

@@ -81,15 +81,21 @@ class Session:
         k = self.step_count if k is None else k
         return self.frame_every > 0 and (k + 1) % self.frame_every == 0
 
-    def step(self, force=None, stream=None):
-        """One lattice step (+ particles); frame work on frame steps. Asynchronous."""
+    def step(self, force=None, stream=None, events=None):
+        """One lattice step (+ particles); frame work on frame steps. Asynchronous.
+        events: optional (start, end) CUDA events recorded around the step kernel."""
         frame = self.is_frame()
         want_force = frame if force is None else force
-        self.T.step(1, self.uin, self.tau, self.smag, flags=WT_FLAG_FORCES if want_force else 0, stream=stream)
+        main = stream if stream is not None else torch.cuda.current_stream(self.T.device)
+        if events is not None:
+            events[0].record(main)
+        self.T.step(1, self.uin, self.tau, self.smag, flags=WT_FLAG_FORCES if want_force else 0, stream=main)
+        if events is not None:
+            events[1].record(main)
         if self.num_p:
-            self.advect(stream)
+            self.advect(main)
         if frame:
-            self.emit_frame(stream)
+            self.emit_frame(main)
         elif want_force:
             self.forces_host.copy_(self.T.sums, non_blocking=True)
         self.step_count += 1
@@ -108,6 +114,7 @@ class Session:
 
     def particles_by_id(self):
         """(pos, spd) on the host in particle-id order."""
+        torch.cuda.synchronize(self.T.device)
         ids = self.ids.cpu().numpy().astype(np.int64)
         pos, spd = np.empty((self.num_p, 4), np.float32), np.empty(self.num_p, np.float32)
         pos[ids] = self.pos.cpu().numpy()

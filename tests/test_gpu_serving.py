@@ -71,3 +71,24 @@ def test_preemption_keeps_results_exact(gpu):
         lv = srv.finished[r.sid]
         assert lv.done_steps == r.steps and len(lv.forces) == r.steps
         assert np.array_equal(np.array(lv.forces).view(np.uint64), _alone(r).view(np.uint64)), r.sid
+
+
+def test_tick_target_bounds_the_active_set(gpu):
+    """With a tick-time target (fixed rate estimate, so the test is deterministic)
+    the active set never exceeds the target's cost units unless it is a single
+    session, everything completes, and every force stream is still exact."""
+    from windtunnel.serving import Server, TickBudget
+    reqs = _requests()
+    units = [TickBudget.units(r.dims, r.form) for r in reqs]
+    # room for about two average sessions per tick
+    tb = TickBudget(1.0, sec_per_unit=1e-3 / (2.0 * float(np.mean(units))), adapt=False)
+    srv = Server(tick_budget=tb)
+    st = srv.run(reqs)
+    assert st['sessions_done'] == len(reqs)
+    limit = tb.target / tb.spu
+    assert st['max_tick_units'] <= max(limit, max(units)) + 1e-6
+    assert st['max_tick_units'] < sum(units)            # the target really held sessions back
+    assert max(srv.queue_delays) > 0.0
+    for r in reqs:
+        lv = srv.finished[r.sid]
+        assert np.array_equal(np.array(lv.forces).view(np.uint64), _alone(r).view(np.uint64)), r.sid

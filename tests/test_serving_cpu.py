@@ -23,3 +23,27 @@ def test_mixed_workload_shape():
     assert [r.form for r in reqs[:4]] == ['fp32', 'fp16', 'fp32', 'fp16']
     again = mixed_workload(seed=3, duration=600.0, rate=2.0)
     assert [r.dims for r in again] == [r.dims for r in reqs]
+
+
+def test_tick_budget_admission_rule():
+    from windtunnel.serving import FORM_COST, TickBudget
+    tb = TickBudget(10.0, sec_per_unit=1e-9, adapt=False)        # 10 ms = 1e7 units
+    assert tb.units((100, 100, 100), 'fp32') == 1e6
+    assert tb.units((100, 100, 100), 'fp16') == 1e6 * 93 / 169
+    assert FORM_COST['fp64'] == 321 / 169
+    assert tb.admits(0.0, 5e8)                                    # an idle GPU takes any head
+    assert tb.admits(9e6, 1e6) and not tb.admits(9e6, 1.5e6)
+    tb.observe(0.02, 1e7)                                         # adapt=False: unchanged
+    assert tb.spu == 1e-9
+
+
+def test_tick_budget_learns_median_rate():
+    from windtunnel.serving import TickBudget
+    tb = TickBudget(50.0, sec_per_unit=1.0, window=5)
+    for dt in (0.010, 0.011, 0.5, 0.009, 0.010):                  # one outlier (a rebuild)
+        tb.observe(dt, 1e7)
+    assert abs(tb.spu - 1.0e-9) < 1e-15                           # median, not mean
+    tb.observe(0.0, 1e7)                                          # empty samples are ignored
+    tb.observe(0.01, 0.0)
+    assert len(tb.samples) == 5
+    assert tb.predicted(2e7) == 2e7 * tb.spu

@@ -3,6 +3,10 @@ cells, 200-5000 steps, half FP32 / half FP16-deviation storage, forces every
 step, a slice every 20 steps), admission capped by HBM, continuous batching.
 
     python -m windtunnel.serve_bench [--duration 600] [--rate 2] [--seed 0] [--graphs] [--time-limit S]
+                                     [--tick-target-ms T]
+
+--tick-target-ms holds the queue while the predicted tick time (every active
+session's step) would exceed T, trading queueing delay for step latency.
 
 Prints one JSON line: p50/p99 step latency, queueing delay, aggregate MLUPS.
 """
@@ -15,10 +19,11 @@ import torch
 from .serving import Server, mixed_workload
 
 
-def run(duration=600.0, rate=2.0, seed=0, graphs=False, time_limit=None, budget_gb=None, progress=True):
+def run(duration=600.0, rate=2.0, seed=0, graphs=False, time_limit=None, budget_gb=None, progress=True,
+        tick_target_ms=None):
     reqs = mixed_workload(seed=seed, duration=duration, rate=rate)
     budget = None if budget_gb is None else int(budget_gb * 1e9)
-    srv = Server(budget_bytes=budget, use_graphs=graphs)
+    srv = Server(budget_bytes=budget, use_graphs=graphs, tick_target_ms=tick_target_ms)
 
     def prog(s, t, nact, nwait, npend):
         print(json.dumps({'t': round(t, 1), 'active': nact, 'waiting': nwait, 'pending': npend,
@@ -39,8 +44,10 @@ def main(argv=None):
     ap.add_argument('--graphs', action='store_true')
     ap.add_argument('--time-limit', type=float, default=None)
     ap.add_argument('--budget-gb', type=float, default=None)
+    ap.add_argument('--tick-target-ms', type=float, default=None)
     args = ap.parse_args(argv)
-    st = run(args.duration, args.rate, args.seed, args.graphs, args.time_limit, args.budget_gb)
+    st = run(args.duration, args.rate, args.seed, args.graphs, args.time_limit, args.budget_gb,
+             tick_target_ms=args.tick_target_ms)
     print(json.dumps(st), flush=True)
     return 0
 

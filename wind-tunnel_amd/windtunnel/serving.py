@@ -8,7 +8,9 @@ loop that keeps many tunnels of uneven size resident in HBM:
   host thread pool while the GPU works;
 * admission: the head of the queue is admitted as soon as its device footprint
   (exact bytes of its state, the generalisation of app.py:72-76) fits in the
-  memory budget; sessions arriving while others run join at the next tick;
+  memory budget and, with a tick-time target, as long as the predicted tick
+  time stays under it (TickBudget); sessions arriving while others run join at
+  the next tick;
 * one tick = one lattice step of every active session: one batched launch per
   storage form (all tunnels of that form in a single kernel launch, or a
   captured graph replay of two steps), forces of every session reduced in
@@ -35,6 +37,40 @@ from ._lib import WT_FLAG_FORCES
 from .batch import Batch
 from .session import FIELD_RANGES
 from .tunnel import Tunnel, state_bytes
+
+
+# Relative cost of one cell update per storage form: the step is bound by HBM
+# traffic, so cost follows bytes per cell (169 FP32, 93 FP16/BF16, 321 FP64).
+FORM_COST = {'fp32': 1.0, 'fp16': 93 / 169, 'bf16': 93 / 169, 'fp64': 321 / 169}
+
+
+class TickBudget:
+    """Admission by predicted tick time. A tick advances every active session one
+    step, so it lasts about sum(cells * FORM_COST) * seconds-per-unit; admitting a
+    session adds its share to every later tick. The rate is learned as the median
+    of recent ticks (time between consecutive results / cost units of the tick)."""
+
+    def __init__(self, target_ms, sec_per_unit=1.0 / 30e9, adapt=True, window=32):
+        self.target = float(target_ms) / 1e3
+        self.spu = float(sec_per_unit)
+        self.adapt = adapt
+        self.samples = collections.deque(maxlen=window)
+
+    @staticmethod
+    def units(dims, form):
+        return float(np.prod(dims)) * FORM_COST[form]
+
+    def observe(self, seconds, units):
+        if self.adapt and seconds > 0 and units > 0:
+            self.samples.append(seconds / units)
+            self.spu = float(np.median(self.samples))
+
+    def predicted(self, units):
+        return units * self.spu
+
+    def admits(self, active_units, new_units):
+        """An idle GPU always takes the head of the queue (no starvation)."""
+        return active_units <= 0 or self.predicted(active_units + new_units) <= self.target
 
 
 @dataclasses.dataclass
@@ -71,7 +107,7 @@ class Live:
 
 class Server:
     def __init__(self, budget_bytes=None, use_graphs=False, keep_forces=True, geometry_workers=4,
-                 device='cuda', preempt_after=None):
+                 device='cuda', preempt_after=None, tick_target_ms=None, tick_budget=None):
         self.device = torch.device(device)
         if budget_bytes is None:
             total = torch.cuda.get_device_properties(self.device).total_memory
@@ -82,6 +118,8 @@ class Server:
         # evict the session with the most remaining steps to host memory when the
         # queue head has waited this long for memory (None: never)
         self.preempt_after = preempt_after
+        # hold the queue while the predicted tick time would exceed the target (None: memory only)
+        self.tick_budget = tick_budget or (TickBudget(tick_target_ms) if tick_target_ms else None)
         self.pool = concurrent.futures.ThreadPoolExecutor(max_workers=geometry_workers)
         self.reset_stats()
 
@@ -93,6 +131,7 @@ class Server:
         self.peak_bytes = 0
         self.finished = {}
         self.evictions = 0
+        self.max_tick_units = 0.0
 
     # ---- geometry -----------------------------------------------------------------
     @staticmethod
@@ -110,17 +149,22 @@ class Server:
         batches = {}                      # form -> Batch
         dirty = set()
         used = 0
-        inflight = collections.deque()    # (event, [(live, step_index)], {form: host sums})
+        units = 0.0                       # cost units of the active set (TickBudget)
+        inflight = collections.deque()    # (event, members, {form: host sums}, cost units)
         t0 = clock()
         last_progress = t0
+        last_retire = [None]              # time of the previous retirement while the GPU was busy
 
         def now():
             return clock() - t0
 
         def retire(entry):
-            ev, members, host_sums = entry
+            ev, members, host_sums, tick_units = entry
             ev.synchronize()
             t = now()
+            if self.tick_budget is not None and last_retire[0] is not None:
+                self.tick_budget.observe(t - last_retire[0], tick_units)
+            last_retire[0] = t
             for form, lives in members.items():
                 hs = host_sums.get(form)
                 hs = hs.numpy() if hs is not None else None
@@ -146,11 +190,14 @@ class Server:
                 resumed = isinstance(item, Live)
                 r = item.req if resumed else item
                 need = state_bytes(r.dims, r.form) + 4 * int(np.prod(r.dims))
+                if self.tick_budget is not None and not self.tick_budget.admits(units, TickBudget.units(r.dims, r.form)):
+                    break
                 if used + need > self.budget:
                     if self.preempt_after is not None and not resumed and now() - r.arrival > self.preempt_after:
                         victim = self._pick_victim(active)
                         if victim is not None:
                             used -= self._evict(victim, active, inflight, retire, dirty)
+                            units -= TickBudget.units(victim.req.dims, victim.req.form)
                             waiting.append(victim)
                             continue
                     break
@@ -174,12 +221,14 @@ class Server:
                 if lv.slice_host is None:
                     lv.slice_host = torch.zeros(2 * nx * ny, dtype=torch.uint8).pin_memory()
                 used += need
+                units += TickBudget.units(r.dims, r.form)
                 self.peak_bytes = max(self.peak_bytes, used)
                 active.setdefault(r.form, []).append(lv)
                 dirty.add(r.form)
             if not any(active.values()):
                 while inflight:
                     retire(inflight.popleft())
+                last_retire[0] = None         # idle time is not tick time
                 if not pending and not waiting:
                     break
                 time.sleep(0.0005)
@@ -235,8 +284,10 @@ class Server:
                         lv.slices += 1
             ev = torch.cuda.Event()
             ev.record()
-            for r in results:
-                inflight.append((ev, members, r))
+            self.max_tick_units = max(self.max_tick_units, units)
+            for i, r in enumerate(results):
+                # one event covers both steps of a graph tick: its time goes with the first entry
+                inflight.append((ev, members, r, units * nsteps if i == 0 else 0.0))
             while len(inflight) > 2 * nsteps:
                 retire(inflight.popleft())
             # retire finished sessions once their last tick is done
@@ -249,6 +300,7 @@ class Server:
                 for lv in fin:
                     lives.remove(lv)
                     used -= state_bytes(lv.req.dims, lv.req.form) + 4 * int(np.prod(lv.req.dims))
+                    units -= TickBudget.units(lv.req.dims, lv.req.form)
                     self.finished[lv.req.sid] = lv
                     lv.tunnel = None
                     lv.slice_dev = None
@@ -295,6 +347,9 @@ class Server:
             'queue_delay_s': {'p50': float(np.percentile(qd, 50)), 'p99': float(np.percentile(qd, 99)),
                               'max': float(qd.max())},
             'peak_resident_gb': self.peak_bytes / 1e9, 'evictions': self.evictions,
+            'max_tick_units': self.max_tick_units,
+            'tick_target_ms': self.tick_budget.target * 1e3 if self.tick_budget else None,
+            'sec_per_unit': self.tick_budget.spu if self.tick_budget else None,
         }
 
 

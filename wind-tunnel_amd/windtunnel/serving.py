@@ -18,6 +18,9 @@ loop that keeps many tunnels of uneven size resident in HBM:
 * the host keeps at most one tick in flight beyond the one it retires, so
   results stream back while the next tick runs;
 * a session that reaches its step count is retired and its memory freed.
+Membership changes never wait for the GPU: all work is on one stream, so a
+freed buffer is reused only by later work, and every in-flight tick keeps the
+batches (descriptors, graphs, tunnels) it launched alive until it is retired.
 
 Each session's arithmetic is independent of the mix it runs in: the batched
 kernel computes every cell exactly as a single-tunnel launch does and the FP64
@@ -150,7 +153,7 @@ class Server:
         dirty = set()
         used = 0
         units = 0.0                       # cost units of the active set (TickBudget)
-        inflight = collections.deque()    # (event, members, {form: host sums}, cost units)
+        inflight = collections.deque()    # (event, members, {form: host sums}, cost units, batches)
         t0 = clock()
         last_progress = t0
         last_retire = [None]              # time of the previous retirement while the GPU was busy
@@ -159,7 +162,7 @@ class Server:
             return clock() - t0
 
         def retire(entry):
-            ev, members, host_sums, tick_units = entry
+            ev, members, host_sums, tick_units, _ = entry
             ev.synchronize()
             t = now()
             if self.tick_budget is not None and last_retire[0] is not None:
@@ -236,8 +239,6 @@ class Server:
             # (re)build the batch of every form whose membership changed
             for form in list(dirty):
                 lives = active.get(form, [])
-                while inflight:              # old descriptors stay valid until retired
-                    retire(inflight.popleft())
                 if lives:
                     bt = Batch([lv.tunnel for lv in lives], [(lv.req.uin, lv.req.tau, lv.req.smag) for lv in lives])
                     if self.use_graphs:
@@ -285,9 +286,10 @@ class Server:
             ev = torch.cuda.Event()
             ev.record()
             self.max_tick_units = max(self.max_tick_units, units)
+            launched = [b for _, _, b in groups]
             for i, r in enumerate(results):
                 # one event covers both steps of a graph tick: its time goes with the first entry
-                inflight.append((ev, members, r, units * nsteps if i == 0 else 0.0))
+                inflight.append((ev, members, r, units * nsteps if i == 0 else 0.0, launched))
             while len(inflight) > 2 * nsteps:
                 retire(inflight.popleft())
             # retire finished sessions once their last tick is done
@@ -295,8 +297,7 @@ class Server:
                 fin = [lv for lv in lives if lv.issued_steps >= lv.req.steps]
                 if not fin:
                     continue
-                while inflight:
-                    retire(inflight.popleft())
+                # its last results are still retired into lv (the in-flight entries hold it)
                 for lv in fin:
                     lives.remove(lv)
                     used -= state_bytes(lv.req.dims, lv.req.form) + 4 * int(np.prod(lv.req.dims))

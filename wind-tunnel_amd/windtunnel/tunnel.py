@@ -83,9 +83,13 @@ class Tunnel:
         mask = np.ascontiguousarray(np.asarray(mask).reshape(-1), dtype=np.uint8)
         if mask.size != self.n:
             raise ValueError(f'solid mask has {mask.size} cells, grid has {self.n}')
-        dev = torch.from_numpy(mask).to(self.device, non_blocking=False)
-        check(self.lib.wt_set_solid(self.gp, _ptr(dev), stream_handle(stream)), 'wt_set_solid')
-        torch.cuda.current_stream().synchronize() if stream is None else stream.synchronize()
+        # staged through pinned memory so the host never waits: the pinned block and the
+        # device copy are released to their caching allocators only after the stream
+        # has used them
+        s = stream if stream is not None else torch.cuda.current_stream(self.device)
+        with torch.cuda.stream(s):
+            dev = torch.from_numpy(mask).pin_memory().to(self.device, non_blocking=True)
+            check(self.lib.wt_set_solid(self.gp, _ptr(dev), stream_handle(s)), 'wt_set_solid')
 
     # ---- hot path ----------------------------------------------------------
     def step(self, nsteps, uin, tau, smag, flags=0, stream=None):

@@ -377,8 +377,15 @@ __device__ __forceinline__ void step_cell_general(const StepDesc& d, const typen
 // allocation so that every scalar offset is non-negative; buffers beyond 32-bit
 // byte offsets use 64-bit global addresses instead.
 constexpr int BUF_GFX9_WORD3 = 0x00020000;  // raw buffer, gfx9-family resource word 3
-constexpr int CPOL_LD = WT_LOAD_NT ? 2 : 0;   // non-temporal cache policy bit
-constexpr int CPOL_ST = WT_STORE_NT ? 2 : 0;
+// cache-policy word of the raw-buffer loads and stores (gfx950: bit 0 sc0, bit 1 nt, bit 4 sc1)
+#ifndef WT_CPOL_LD
+#define WT_CPOL_LD (WT_LOAD_NT ? 2 : 0)
+#endif
+#ifndef WT_CPOL_ST
+#define WT_CPOL_ST (WT_STORE_NT ? 2 : 0)
+#endif
+constexpr int CPOL_LD = WT_CPOL_LD;
+constexpr int CPOL_ST = WT_CPOL_ST;
 
 template <typename E> struct RawIO;
 template <> struct RawIO<float> {

@@ -40,25 +40,30 @@ __device__ __forceinline__ float4 lerp4(float4 a, float4 b, float t) {
   return make_float4(a.x * s + b.x * t, a.y * s + b.y * t, a.z * s + b.z * t, a.w * s + b.w * t);
 }
 
-// Trilinear sample of the macroscopic field, lbm.py:206-230.
-__device__ float4 sample_mac(const float4* __restrict__ mac, float px, float py, float pz, int nx, int ny,
-                             int nz) {
+// Trilinear sample of the macroscopic field, lbm.py:206-230. The 8 corner
+// values are kept between calls: the second RK2 sample of a particle usually
+// lies in the cell of the first, and then reuses them (same values, so the same
+// result) instead of gathering them again.
+struct Corners {
+  int ix, iy, iz;
+  float4 c[8];   // (x, y, z) offsets 000 100 010 110 001 101 011 111
+};
+
+__device__ __forceinline__ float4 sample_mac(const float4* __restrict__ mac, float px, float py, float pz, int nx,
+                                             int ny, int nz, Corners& k, bool have) {
   const float qx = clampf(px, 0.0f, (float)nx - 1.001f);
   const float qy = clampf(py, 0.0f, (float)ny - 1.001f);
   const float qz = clampf(pz, 0.0f, (float)nz - 1.001f);
   const float bx = floorf(qx), by = floorf(qy), bz = floorf(qz);
   const float tx = qx - bx, ty = qy - by, tz = qz - bz;
   const int ix = (int)bx, iy = (int)by, iz = (int)bz;
-  const float4 c000 = mac[cidx(ix, iy, iz, nx, ny, nz)];
-  const float4 c100 = mac[cidx(ix + 1, iy, iz, nx, ny, nz)];
-  const float4 c010 = mac[cidx(ix, iy + 1, iz, nx, ny, nz)];
-  const float4 c110 = mac[cidx(ix + 1, iy + 1, iz, nx, ny, nz)];
-  const float4 c001 = mac[cidx(ix, iy, iz + 1, nx, ny, nz)];
-  const float4 c101 = mac[cidx(ix + 1, iy, iz + 1, nx, ny, nz)];
-  const float4 c011 = mac[cidx(ix, iy + 1, iz + 1, nx, ny, nz)];
-  const float4 c111 = mac[cidx(ix + 1, iy + 1, iz + 1, nx, ny, nz)];
-  const float4 a = lerp4(lerp4(c000, c100, tx), lerp4(c010, c110, tx), ty);
-  const float4 b = lerp4(lerp4(c001, c101, tx), lerp4(c011, c111, tx), ty);
+  if (!(have && ix == k.ix && iy == k.iy && iz == k.iz)) {
+    k.ix = ix; k.iy = iy; k.iz = iz;
+#pragma unroll
+    for (int c = 0; c < 8; c++) k.c[c] = mac[cidx(ix + (c & 1), iy + ((c >> 1) & 1), iz + (c >> 2), nx, ny, nz)];
+  }
+  const float4 a = lerp4(lerp4(k.c[0], k.c[1], tx), lerp4(k.c[2], k.c[3], tx), ty);
+  const float4 b = lerp4(lerp4(k.c[4], k.c[5], tx), lerp4(k.c[6], k.c[7], tx), ty);
   return lerp4(a, b, tz);
 }
 
@@ -101,9 +106,10 @@ __global__ __launch_bounds__(ABLOCK) void k_advect(AdvectArgs a) {
   const uint32_t i = a.ids ? a.ids[slot] : slot;
   const float4 p = a.pos[slot];
   float qx = p.x, qy = p.y, qz = p.z;
-  const float4 m1 = sample_mac(a.mac, qx, qy, qz, a.nx, a.ny, a.nz);
+  Corners k;
+  const float4 m1 = sample_mac(a.mac, qx, qy, qz, a.nx, a.ny, a.nz, k, false);
   const float h = 0.5f * a.dt;
-  const float4 m2 = sample_mac(a.mac, qx + h * m1.x, qy + h * m1.y, qz + h * m1.z, a.nx, a.ny, a.nz);
+  const float4 m2 = sample_mac(a.mac, qx + h * m1.x, qy + h * m1.y, qz + h * m1.z, a.nx, a.ny, a.nz, k, true);
   qx = qx + a.dt * m2.x;
   qy = qy + a.dt * m2.y;
   qz = qz + a.dt * m2.z;

@@ -377,6 +377,10 @@ __device__ __forceinline__ void step_cell_general(const StepDesc& d, const typen
 // allocation so that every scalar offset is non-negative; buffers beyond 32-bit
 // byte offsets use 64-bit global addresses instead.
 constexpr int BUF_GFX9_WORD3 = 0x00020000;  // raw buffer, gfx9-family resource word 3
+// Directions in ascending memory order of their upstream rows (z-1 rows first,
+// then y-1, the own row, y+1, z+1; x+1 before x-1 within a row): issuing the 19
+// loads in this order measured 0.5-0.7 % faster than direction order.
+constexpr int LOAD_ORDER[Q] = {15, 11, 5, 14, 18, 7, 3, 10, 1, 0, 2, 9, 4, 8, 17, 13, 6, 12, 16};
 // cache-policy word of the raw-buffer loads and stores (gfx950: bit 0 sc0, bit 1 nt, bit 4 sc1)
 #ifndef WT_CPOL_LD
 #define WT_CPOL_LD (WT_LOAD_NT ? 2 : 0)
@@ -430,14 +434,20 @@ __device__ __forceinline__ void step_cell_fast(const StepDesc& d, const typename
     R f[Q];
     auto global_loads = [&]() {
 #pragma unroll
-      for (int i = 0; i < Q; i++) f[i] = S::load(ld_stream(fin + (pc + pshift(i, nx, nxy))), i);
+      for (int k = 0; k < Q; k++) {
+        const int i = LOAD_ORDER[k];
+        f[i] = S::load(ld_stream(fin + (pc + pshift(i, nx, nxy))), i);
+      }
     };
     if constexpr (sizeof(E) <= 4) {
       if (!d.global_addr) {
         const __amdgpu_buffer_rsrc_t rin = __builtin_amdgcn_make_buffer_rsrc(
             (void*)((const char*)fin - bias), (short)0, (int)((uint32_t)Q * n * ES + bias), BUF_GFX9_WORD3);
 #pragma unroll
-        for (int i = 0; i < Q; i++) f[i] = S::load(RawIO<E>::ld(rin, bo, pshift(i, nx, nxy) * ES + bias), i);
+        for (int k = 0; k < Q; k++) {
+          const int i = LOAD_ORDER[k];
+          f[i] = S::load(RawIO<E>::ld(rin, bo, pshift(i, nx, nxy) * ES + bias), i);
+        }
       } else {
         global_loads();
       }

@@ -94,6 +94,28 @@ def test_step_bit_exact_with_forces_and_mass(gpu, name, form):
         np.testing.assert_allclose(sums[3], o.mass, rtol=1e-12)
 
 
+# ragged and minimum grids: rows shorter than a wave, a single interior cell, odd sizes
+@pytest.mark.parametrize('dims', [(3, 3, 3), (5, 4, 3), (3, 7, 5), (130, 3, 4), (67, 9, 5)])
+@pytest.mark.parametrize('name,form', [('fp32', O.FP32), ('fp16', O.FP16)])
+def test_step_bit_exact_ragged_sizes(gpu, dims, name, form):
+    from windtunnel._lib import WT_FLAG_FORCES, WT_FLAG_MASS
+    n = dims[0] * dims[1] * dims[2]
+    rng = np.random.default_rng(n)
+    mask = np.where(rng.random(n) < 0.15, 1, 0).astype(np.uint8)
+    mask[rng.random(n) < 0.05] = 2
+    uin, tau, smag = 0.05, 0.53, 0.16
+    t = _tunnel(dims, name, mask, uin)
+    o = _oracle(dims, form, mask, uin)
+    t.step(6, uin, tau, smag, flags=WT_FLAG_FORCES | WT_FLAG_MASS)
+    o.step(6, uin, tau, smag)
+    torch.cuda.synchronize()
+    assert _raw_equal(t.raw_populations().cpu().numpy(), o.raw()), dims
+    assert _raw_equal(t.mac.cpu().numpy(), o.mac.astype(np.float32)), dims
+    sums = t.sums.cpu().numpy()
+    np.testing.assert_allclose(sums[:3], o.forces(), rtol=1e-12, atol=1e-15)
+    np.testing.assert_allclose(sums[3], o.mass, rtol=1e-12)
+
+
 def test_config1_sphere_100_steps(gpu):
     """Config 1: 128x64x64, sphere D=16 at (40,32,32), uin 0.05, tau 0.52, Cs 0.17."""
     from windtunnel._lib import WT_FLAG_FORCES

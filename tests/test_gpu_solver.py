@@ -38,6 +38,7 @@ def test_solver_frame_matches_oracle(gpu):
     s.set_surface(verts.astype(np.float32), norms.astype(np.float32))
     p = dict(P, seedCy=13.0, seedCz=17.0, seedHy=8.0, seedHz=8.0, maxAge=300.0)
     s.set_particles(2000, p)
+    s.sort_every = 2                              # re-sort the particle slots inside the test
 
     o = O.OracleTunnel(dims, O.FP32)
     o.init(p['uIn'])

@@ -17,6 +17,9 @@ Differences from the reference, all deliberate:
   * Optional keyword `storage` ('fp32' default, 'fp16', 'bf16', 'fp64') and
     `seed` (for set_particles; the reference's generator is unseeded,
     lbm.py:430).
+  * Particles live in cell-ordered slots (re-sorted every `sort_every`
+    advections), each slot carrying its particle id; frame() still returns
+    them in id order and every particle's values are unchanged.
 """
 import os
 import sys
@@ -96,6 +99,9 @@ class Solver:
         self.seed = 1
         self._rng_seed = seed
         self._forces_fresh = False
+        self.sort_every = 8       # advections between re-sorts of the particle slots
+        self._advects = 0
+        self._order_ws = None
 
     @property
     def cur(self):
@@ -132,6 +138,11 @@ class Solver:
         self.num_p = count
         self.pos = torch.from_numpy(d).cuda()
         self.spd = torch.ones(count, dtype=torch.float32, device='cuda')
+        self.ids = torch.arange(count, dtype=torch.int32, device='cuda')
+        self._spare = (torch.empty_like(self.pos), torch.empty_like(self.spd), torch.empty_like(self.ids))
+        self._order = torch.empty(count, dtype=torch.int32, device='cuda')
+        self._order_ws = None
+        self._advects = 0
         self.q16 = torch.zeros(count * 4, dtype=torch.int16, device='cuda')
         self.q8 = torch.zeros(count, dtype=torch.uint8, device='cuda')
 
@@ -147,9 +158,16 @@ class Solver:
         """The particle half of step(): one advection with dt = steps (lbm.py:448-453)."""
         steps = int(steps)
         if self.num_p and steps:
+            if self._advects % self.sort_every == 0:
+                self._order_ws = self.T.particle_order(self.pos, self._order, self._order_ws)
+                pos2, spd2, ids2 = self._spare
+                self.T.particle_gather(self.pos, self.spd, self.ids, self._order, pos2, spd2, ids2)
+                self._spare = (self.pos, self.spd, self.ids)
+                self.pos, self.spd, self.ids = pos2, spd2, ids2
+            self._advects += 1
             self.seed += 1
-            self.T.advect(self.pos, self.spd, float(steps), p['uIn'], p['maxAge'], self.seed,
-                          (p['seedX'], p['seedCy'], p['seedCz'], p['seedHy'], p['seedHz']))
+            self.T.advect_slots(self.pos, self.spd, self.ids, float(steps), p['uIn'], p['maxAge'], self.seed,
+                                (p['seedX'], p['seedCy'], p['seedCz'], p['seedHy'], p['seedHz']))
 
     def set_forces(self, sums):
         """Take the force of the current state from a batched step's FP64 sums
@@ -165,7 +183,7 @@ class Solver:
     def frame(self, p, want_cp):
         out = {}
         if self.num_p:
-            self.T.pack_particles(self.pos, self.spd, self.q16, self.q8)
+            self.T.pack_particles_slots(self.pos, self.spd, self.ids, self.q16, self.q8)
             out['parts16'] = self.q16.cpu().numpy().view(np.uint16)
             out['parts8'] = self.q8.cpu().numpy()
         sl = p.get('slice')

@@ -140,6 +140,36 @@ def test_config1_sphere_100_steps(gpu):
     assert _raw_equal(t.mac.cpu().numpy(), o.mac)
 
 
+def test_config1_time_averaged_cd_cl_500_steps(gpu):
+    """Config 1 as specified: 500 steps from rest, momentum-exchange force every step,
+    Cd, Cl = F / (0.5 uin^2 A) with A the frontal count (197 cells), averaged over
+    steps 251-500. Stated tolerance: 0.1 % relative on the averages (the two sides
+    agree to FP64 summation order, ~1e-12)."""
+    from windtunnel import geometry
+    from windtunnel._lib import WT_FLAG_FORCES
+    dims = (128, 64, 64)
+    mask, counts = geometry.sphere(dims, (40, 32, 32), 16)
+    assert counts['frontal'] == 197
+    q = 0.5 * 0.05 ** 2 * counts['frontal']
+    t = _tunnel(dims, 'fp32', mask, 0.05)
+    o = _oracle(dims, O.FP32, mask, 0.05)
+    fg = torch.zeros((500, 4), dtype=torch.float64, device='cuda')
+    fc = np.zeros((500, 3))
+    for k in range(500):
+        t.step(1, 0.05, 0.52, 0.17, flags=WT_FLAG_FORCES)
+        fg[k].copy_(t.sums)
+        o.step(1, 0.05, 0.52, 0.17)
+        fc[k] = o.forces()
+    fg = fg.cpu().numpy()[:, :3]
+    cd_g, cl_g = fg[250:, 0].mean() / q, fg[250:, 1].mean() / q
+    cd_c, cl_c = fc[250:, 0].mean() / q, fc[250:, 1].mean() / q
+    print(f'config-1 500 steps: Cd {cd_g:.6f} (oracle {cd_c:.6f}), Cl {cl_g:.3e} (oracle {cl_c:.3e})')
+    assert np.isfinite(cd_g) and 0.1 < cd_g < 10.0
+    assert abs(cd_g - cd_c) <= 1e-3 * abs(cd_c)
+    assert abs(cl_g - cl_c) <= 1e-3 * max(abs(cl_c), abs(cd_c))
+    np.testing.assert_allclose(fg, fc, rtol=1e-10, atol=1e-13)
+
+
 def test_fused_force_equals_standalone(gpu):
     from windtunnel._lib import WT_FLAG_FORCES
     dims = (64, 24, 24)

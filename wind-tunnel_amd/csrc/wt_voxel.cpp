@@ -33,53 +33,96 @@ static inline double js_round(double x) {
 
 // Flood fill of non-marked cells from the domain boundary, then areas and the
 // ground tag (voxelize.js:73-114). `vox` holds the marked cells on entry;
-// `out` receives the final mask.
+// `out` receives the final mask. Unreached open cells become solid.
+//
+// The fill works on runs: maximal x-intervals of open cells in one (y, z) row.
+// Two runs are 6-connected exactly when they lie in rows adjacent in y or z and
+// their x-intervals overlap (runs of one row are separated by marked cells), so
+// a breadth-first search over runs reaches the same cells as one over cells,
+// touching each cell only to build the runs and to write the result.
 static void close_and_count(const std::vector<uint8_t>& vox, int nx, int ny, int nz, int ground,
                             uint8_t* out, int64_t counts[3]) {
-  const int64_t n = (int64_t)nx * ny * nz;
-  const int64_t nxy = (int64_t)nx * ny;
-  std::vector<uint8_t> seen(n, 0);
-  std::vector<int64_t> stack;
-  stack.reserve(1 << 16);
-  for (int z = 0; z < nz; z++)
-    for (int y = 0; y < ny; y++)
-      for (int x = 0; x < nx; x++) {
-        if (x == 0 || y == 0 || z == 0 || x == nx - 1 || y == ny - 1 || z == nz - 1) {
-          const int64_t i = x + (int64_t)nx * (y + (int64_t)ny * z);
-          if (!vox[i]) { seen[i] = 1; stack.push_back(i); }
-        }
-      }
-  while (!stack.empty()) {
-    const int64_t i = stack.back();
-    stack.pop_back();
-    const int64_t x = i % nx, y = (i / nx) % ny, z = i / nxy;
-    if (x > 0 && !vox[i - 1] && !seen[i - 1]) { seen[i - 1] = 1; stack.push_back(i - 1); }
-    if (x < nx - 1 && !vox[i + 1] && !seen[i + 1]) { seen[i + 1] = 1; stack.push_back(i + 1); }
-    if (y > 0 && !vox[i - nx] && !seen[i - nx]) { seen[i - nx] = 1; stack.push_back(i - nx); }
-    if (y < ny - 1 && !vox[i + nx] && !seen[i + nx]) { seen[i + nx] = 1; stack.push_back(i + nx); }
-    if (z > 0 && !vox[i - nxy] && !seen[i - nxy]) { seen[i - nxy] = 1; stack.push_back(i - nxy); }
-    if (z < nz - 1 && !vox[i + nxy] && !seen[i + nxy]) { seen[i + nxy] = 1; stack.push_back(i + nxy); }
-  }
-  int64_t count = 0;
-  for (int64_t i = 0; i < n; i++) {
-    out[i] = seen[i] ? 0 : 1;
-    count += out[i];
-  }
-  int64_t frontal = 0;
-  for (int k = 0; k < nz; k++)
-    for (int j = 0; j < ny; j++) {
-      const int64_t base = (int64_t)nx * (j + (int64_t)ny * k);
-      for (int x = 0; x < nx; x++)
-        if (out[base + x]) { frontal++; break; }
+  const int64_t nrows = (int64_t)ny * nz;
+  std::vector<int64_t> row_start(nrows + 1, 0);
+  std::vector<int32_t> rx0, rx1;
+  rx0.reserve(nrows * 2);
+  rx1.reserve(nrows * 2);
+  for (int64_t r = 0; r < nrows; r++) {
+    row_start[r] = (int64_t)rx0.size();
+    const uint8_t* v = vox.data() + r * nx;
+    int x = 0;
+    while (x < nx) {
+      while (x < nx && v[x]) x++;
+      if (x >= nx) break;
+      const int a = x;
+      while (x < nx && !v[x]) x++;
+      rx0.push_back(a);
+      rx1.push_back(x - 1);
     }
+  }
+  row_start[nrows] = (int64_t)rx0.size();
+  const int64_t nruns = (int64_t)rx0.size();
+  std::vector<uint8_t> reached(nruns, 0);
+  std::vector<int64_t> queue;
+  queue.reserve(1 << 16);
+  // seeds: every run in a boundary row, and runs touching x = 0 or x = nx - 1
+  for (int z = 0; z < nz; z++)
+    for (int y = 0; y < ny; y++) {
+      const int64_t r = y + (int64_t)ny * z;
+      const bool edge_row = y == 0 || z == 0 || y == ny - 1 || z == nz - 1;
+      for (int64_t q = row_start[r]; q < row_start[r + 1]; q++)
+        if (edge_row || rx0[q] == 0 || rx1[q] == nx - 1) { reached[q] = 1; queue.push_back(q); }
+    }
+  // runs of row nr overlapping [a, b]: the first with x1 >= a, then while x0 <= b
+  auto visit_row = [&](int64_t nr, int a, int b) {
+    int64_t lo = row_start[nr], hi = row_start[nr + 1];
+    while (lo < hi) {
+      const int64_t mid = (lo + hi) >> 1;
+      if (rx1[mid] < a) lo = mid + 1; else hi = mid;
+    }
+    for (int64_t q = lo; q < row_start[nr + 1] && rx0[q] <= b; q++)
+      if (!reached[q]) { reached[q] = 1; queue.push_back(q); }
+  };
+  std::vector<int64_t> run_row(nruns);
+  for (int64_t r = 0; r < nrows; r++)
+    for (int64_t q = row_start[r]; q < row_start[r + 1]; q++) run_row[q] = r;
+  for (size_t h = 0; h < queue.size(); h++) {
+    const int64_t q = queue[h];
+    const int64_t r = run_row[q];
+    const int y = (int)(r % ny), z = (int)(r / ny);
+    const int a = rx0[q], b = rx1[q];
+    if (y > 0) visit_row(r - 1, a, b);
+    if (y < ny - 1) visit_row(r + 1, a, b);
+    if (z > 0) visit_row(r - ny, a, b);
+    if (z < nz - 1) visit_row(r + ny, a, b);
+  }
+  // mask, cell count and frontal area (rows with a solid cell), row by row
+  int64_t count = 0, frontal = 0;
+  for (int64_t r = 0; r < nrows; r++) {
+    uint8_t* o = out + r * nx;
+    std::memset(o, 1, (size_t)nx);
+    int64_t open = 0;
+    for (int64_t q = row_start[r]; q < row_start[r + 1]; q++)
+      if (reached[q]) {
+        std::memset(o + rx0[q], 0, (size_t)(rx1[q] - rx0[q] + 1));
+        open += rx1[q] - rx0[q] + 1;
+      }
+    count += nx - open;
+    if (open < nx) frontal++;
+  }
+  // planform area: (x, z) columns with a solid cell at any y
   int64_t planform = 0;
-  for (int k = 0; k < nz; k++)
-    for (int x = 0; x < nx; x++)
-      for (int j = 0; j < ny; j++)
-        if (out[x + (int64_t)nx * (j + (int64_t)ny * k)]) { planform++; break; }
+  std::vector<uint8_t> colf((size_t)nx);
+  for (int z = 0; z < nz; z++) {
+    std::fill(colf.begin(), colf.end(), 0);
+    for (int y = 0; y < ny; y++) {
+      const uint8_t* o = out + (int64_t)nx * (y + (int64_t)ny * z);
+      for (int x = 0; x < nx; x++) colf[x] |= o[x];
+    }
+    for (int x = 0; x < nx; x++) planform += colf[x];
+  }
   if (ground)
-    for (int k = 0; k < nz; k++)
-      for (int x = 0; x < nx; x++) out[x + nxy * k] = 2;
+    for (int z = 0; z < nz; z++) std::memset(out + (int64_t)nx * ny * z, 2, (size_t)nx);
   counts[0] = count;
   counts[1] = frontal;
   counts[2] = planform;

@@ -36,8 +36,11 @@ extern "C" {
  * f[0], f[1]: ping-pong populations, row-blocked SoA: each x-row
  *             r = y + ny*z holds its 19 planes of nx values in turn, so
  *             population i of cell idx = x + nx*r is element
- *             19*nx*r + nx*i + x (the reference's plane i*n + idx,
- *             lbm.py:68, 118, cut into rows). 19*n elements.
+ *             19*nx*r + nx*p[i] + x (the reference's plane i*n + idx,
+ *             lbm.py:68, 118, cut into rows), with plane positions
+ *             p = {0,1,2,3,6,9,12,4,7,8,5,10,13,14,11,15,16,17,18}
+ *             (planes read by the same neighbouring row adjacent).
+ *             19*n elements.
  * cls:        per-cell class byte built by wt_set_solid from the solid mask.
  * mac:        float4 (ux, uy, uz, rho) per cell (lbm.py:406).
  * shell[0,1]: ping-pong snapshot of the macroscopic field on the six layers

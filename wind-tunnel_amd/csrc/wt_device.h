@@ -33,7 +33,7 @@ constexpr int opp(int i) {
 // lattice weight in double, lbm.py:18
 // Population layout: row-blocked SoA. Each x-row r = y + ny*z of nx cells
 // stores its 19 planes of nx values one after another: population i of cell
-// idx = x + nx*r is element 19*nx*r + nx*i + x. A wave's consecutive cells read
+// idx = x + nx*r is element 19*nx*r + nx*ppos(i) + x. A wave's consecutive cells read
 // or write one contiguous run per plane, and a row's 19 runs lie together.
 // Interior cells never pull across their own row in x and y/z neighbours are
 // whole rows away, so population i of the upstream neighbour idx - c_i sits at
@@ -41,10 +41,18 @@ constexpr int opp(int i) {
 __host__ __device__ __forceinline__ uint32_t prow(uint32_t idx, uint32_t row, uint32_t nx) {
   return idx + (uint32_t)(Q - 1) * nx * row;
 }
+// Position of direction i's plane inside a row block. Planes are grouped by the
+// row that reads them (equal (c_y, c_z)): {0, 1, 2}, {3, 7, 10}, {4, 8, 9},
+// {5, 11, 14}, {6, 12, 13}, 15, 16, 17, 18, so the pulls of one wave from one
+// upstream row come from one contiguous stretch of that row's block.
+__host__ __device__ constexpr int ppos(int i) {
+  constexpr int P[Q] = {0, 1, 2, 3, 6, 9, 12, 4, 7, 8, 5, 10, 13, 14, 11, 15, 16, 17, 18};
+  return P[i];
+}
 // pshift in 32-bit wrap-around arithmetic: prow + pshift is the exact element
 // index (< 19 n < 2^32) even where the signed offset is negative.
 __host__ __device__ __forceinline__ uint32_t pshift(int i, uint32_t nx, uint32_t nxy) {
-  return (uint32_t)i * nx - ((uint32_t)cx(i) + (uint32_t)(Q * cy(i)) * nx + (uint32_t)(Q * cz(i)) * nxy);
+  return (uint32_t)ppos(i) * nx - ((uint32_t)cx(i) + (uint32_t)(Q * cy(i)) * nx + (uint32_t)(Q * cz(i)) * nxy);
 }
 
 constexpr double wd(int i) { return i == 0 ? 1.0 / 3.0 : (i <= 6 ? 1.0 / 18.0 : 1.0 / 36.0); }

@@ -328,7 +328,7 @@ __device__ __forceinline__ void step_cell_general(const StepDesc& d, const typen
     const R bsq = (R)1.5 * (bux * bux + buy * buy + buz * buz);
 #pragma unroll
     for (int i = 0; i < Q; i++)
-      st_stream(fout + (pc + (uint32_t)i * (uint32_t)nx), S::store(face_feq<R>(i, br, bux, buy, buz, bsq), i));
+      st_stream(fout + (pc + (uint32_t)ppos(i) * (uint32_t)nx), S::store(face_feq<R>(i, br, bux, buy, buz, bsq), i));
     st_mac(d.mac + idx, (float)bux, (float)buy, (float)buz, (float)br);
     if (flags & WT_FLAG_MASS) acc.m += (double)br;
     return;
@@ -344,7 +344,7 @@ __device__ __forceinline__ void step_cell_general(const StepDesc& d, const typen
       f[0] = S::load(ld_stream(fin + pc), 0);
     } else {
       const bool b = (lk.bb >> i) & 1u;
-      const uint32_t a = pc + (b ? (uint32_t)opp(i) * (uint32_t)nx : pshift(i, nx, nxy));
+      const uint32_t a = pc + (b ? (uint32_t)ppos(opp(i)) * (uint32_t)nx : pshift(i, nx, nxy));
       f[i] = S::load(ld_stream(fin + a), i);
     }
   }
@@ -360,7 +360,7 @@ __device__ __forceinline__ void step_cell_general(const StepDesc& d, const typen
 #pragma unroll
   for (int i = 0; i < Q; i++) {
     st[i] = S::store(out[i], i);
-    st_stream(fout + (pc + (uint32_t)i * (uint32_t)nx), st[i]);
+    st_stream(fout + (pc + (uint32_t)ppos(i) * (uint32_t)nx), st[i]);
   }
   st_mac(d.mac + idx, (float)ux, (float)uy, (float)uz, (float)rho);
   shell_write<R>(shout, x, y, z, ux, uy, uz, rho);
@@ -380,7 +380,7 @@ constexpr int BUF_GFX9_WORD3 = 0x00020000;  // raw buffer, gfx9-family resource 
 // Directions in ascending memory order of their upstream rows (z-1 rows first,
 // then y-1, the own row, y+1, z+1; x+1 before x-1 within a row): issuing the 19
 // loads in this order measured 0.5-0.7 % faster than direction order.
-constexpr int LOAD_ORDER[Q] = {15, 11, 5, 14, 18, 7, 3, 10, 1, 0, 2, 9, 4, 8, 17, 13, 6, 12, 16};
+constexpr int LOAD_ORDER[Q] = {15, 5, 11, 14, 18, 3, 7, 10, 0, 1, 2, 4, 8, 9, 17, 6, 12, 13, 16};
 // cache-policy word of the raw-buffer loads and stores (gfx950: bit 0 sc0, bit 1 nt, bit 4 sc1)
 #ifndef WT_CPOL_LD
 #define WT_CPOL_LD (WT_LOAD_NT ? 2 : 0)
@@ -458,14 +458,14 @@ __device__ __forceinline__ void step_cell_fast(const StepDesc& d, const typename
   }
   auto global_stores = [&]() {
 #pragma unroll
-    for (int i = 0; i < Q; i++) st_stream(fout + (pc + (uint32_t)i * nx), S::store(out[i], i));
+    for (int i = 0; i < Q; i++) st_stream(fout + (pc + (uint32_t)ppos(i) * nx), S::store(out[i], i));
   };
   if constexpr (sizeof(E) <= 4) {
     if (!d.global_addr) {
       const __amdgpu_buffer_rsrc_t rout =
           __builtin_amdgcn_make_buffer_rsrc((void*)fout, (short)0, (int)((uint32_t)Q * n * ES), BUF_GFX9_WORD3);
 #pragma unroll
-      for (int i = 0; i < Q; i++) RawIO<E>::st(rout, bo, (uint32_t)i * nx * ES, S::store(out[i], i));
+      for (int i = 0; i < Q; i++) RawIO<E>::st(rout, bo, (uint32_t)ppos(i) * nx * ES, S::store(out[i], i));
     } else {
       global_stores();
     }
@@ -699,7 +699,7 @@ __global__ __launch_bounds__(BLOCK) void k_force(StepDesc d) {
 #pragma unroll
       for (int i = 1; i < Q; i++) {
         if ((fl >> i) & 1u) {
-          const R v = (R)2.0 * (S::load(f[pc + (uint32_t)i * (uint32_t)nx], i) - (R)wd(i));
+          const R v = (R)2.0 * (S::load(f[pc + (uint32_t)ppos(i) * (uint32_t)nx], i) - (R)wd(i));
           const double dv = (double)v;
           if (cx(i) > 0) a.fx += dv; else if (cx(i) < 0) a.fx -= dv;
           if (cy(i) > 0) a.fy += dv; else if (cy(i) < 0) a.fy -= dv;
@@ -728,8 +728,8 @@ __global__ __launch_bounds__(BLOCK) void k_init(typename Form<FORM>::elem* f0,
     const R v = (R)wd(i) * ((R)1.0 + (R)(3 * cx(i)) * uin + (R)(4.5 * cx(i) * cx(i)) * uin * uin -
                             (R)1.5 * uin * uin);
     const typename S::elem e = S::store(v, i);
-    f0[pc + (uint32_t)i * nx] = e;
-    f1[pc + (uint32_t)i * nx] = e;
+    f0[pc + (uint32_t)ppos(i) * nx] = e;
+    f1[pc + (uint32_t)ppos(i) * nx] = e;
   }
   mac[idx] = make_float4(uin_f, 0.f, 0.f, 1.f);
 }

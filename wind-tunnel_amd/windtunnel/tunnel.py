@@ -15,6 +15,8 @@ from . import _lib, autotune
 from ._lib import Grid, Params, SeedBox, check, stream_handle, FORMS, FORM_NAMES
 
 ELEM_DTYPE = {0: torch.float32, 1: torch.int16, 2: torch.int16, 3: torch.float64}
+# position of direction i's plane inside a row block (include/wt.h)
+PLANE_POS = [0, 1, 2, 3, 6, 9, 12, 4, 7, 8, 5, 10, 13, 14, 11, 15, 16, 17, 18]
 # population weights w_i (server/lbm.py:18)
 W = np.array([1 / 3] + [1 / 18] * 6 + [1 / 36] * 12)
 
@@ -118,7 +120,7 @@ class Tunnel:
         planes in turn); this returns a reordered copy."""
         k = self.cur if which is None else which
         rows = self.f[k].view(ELEM_DTYPE[self.form]).view(self.ny * self.nz, 19, self.nx)
-        return rows.permute(1, 0, 2).reshape(19, self.n)
+        return rows[:, PLANE_POS, :].permute(1, 0, 2).reshape(19, self.n)
 
     def populations(self, which=None, dtype=np.float64):
         """f_i of every cell on the host, shape (19, n), reconstructed from the storage form."""

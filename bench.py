@@ -12,8 +12,12 @@ server/app.py:180 counts them); the same workload with FP16-deviation storage
 is reported beside it. roofline: algorithmic bytes of the step kernel
 (169 B/cell FP32, 93 B/cell FP16: 19 populations read and written, the class
 byte, the 16 B macroscopic store) over its mean launch time, from HIP events on
-the stream the kernel runs on. cpu_baseline: the CPU restatement of the same
-step (oracle/, C++ with OpenMP) timed on this host on a bounded sample.
+the stream the kernel runs on. With FP32 storage each step's launch also
+advects the particles of the previous step (k_step_adv), priced at 40 more
+bytes per particle (id 4, position 16 read + 16 written, speed 4 written).
+Every timed step's particles are advected inside the timed region.
+cpu_baseline: the CPU restatement of the same step (oracle/, C++ with OpenMP)
+timed on this host on a bounded sample.
 """
 import argparse
 import json
@@ -27,6 +31,8 @@ sys.path.insert(0, REPO)
 
 PEAK_HBM_GBS = 8000.0
 BYTES_PER_CELL = {'fp32': 169, 'fp16': 93, 'bf16': 93, 'fp64': 321}
+BYTES_PER_PARTICLE = 40
+FORM_IDS = {'fp32': 0, 'fp16': 1, 'bf16': 2, 'fp64': 3}
 DIMS = (512, 256, 256)
 UIN, TAU, SMAG = 0.05, 0.51, 0.16
 BODY_SEED = 2025
@@ -55,6 +61,7 @@ def run_gpu(form, steps, warmup, mask, counts, lref):
     stream = torch.cuda.current_stream()
     for _ in range(warmup):
         s.step()
+    s.flush()
     torch.cuda.synchronize()
     ev = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in range(steps)]
     is_force = []
@@ -66,21 +73,27 @@ def run_gpu(form, steps, warmup, mask, counts, lref):
     for k in range(steps):
         is_force.append(s.is_frame())
         s.step(stream=stream, events=ev[k])
+    s.flush(stream)                        # the last step's particles (fused sessions)
     t_end.record(stream)
     torch.cuda.synchronize()
     wall = time.perf_counter() - h0
     gpu_ms = t_start.elapsed_time(t_end)
-    plain = [ev[k][0].elapsed_time(ev[k][1]) for k in range(steps) if not is_force[k]]
-    forced = [ev[k][0].elapsed_time(ev[k][1]) for k in range(steps) if is_force[k]]
+    # a fused session's first timed launch is a plain step (nothing pending yet)
+    k0 = 1 if s.fuse else 0
+    plain = [ev[k][0].elapsed_time(ev[k][1]) for k in range(k0, steps) if not is_force[k]]
+    forced = [ev[k][0].elapsed_time(ev[k][1]) for k in range(k0, steps) if is_force[k]]
     step_ms = sum(plain) / max(1, len(plain))
     cd, cl = s.coefficients()
     n = s.n
+    nbytes = BYTES_PER_CELL[form] * n + (BYTES_PER_PARTICLE * s.num_p if s.fuse else 0)
     res = {
         'form': form, 'wall_s': wall, 'gpu_ms': gpu_ms, 'mlups': n * steps / wall / 1e6,
         'ms_per_step': wall * 1e3 / steps, 'step_kernel_ms': step_ms,
         'step_kernel_force_ms': sum(forced) / max(1, len(forced)),
-        'kernel_mlups': n / (step_ms * 1e-3) / 1e6,
-        'achieved_gbs': BYTES_PER_CELL[form] * n / (step_ms * 1e-3) / 1e9,
+        'kernel_mlups': n / (step_ms * 1e-3) / 1e6, 'fused_advect': s.fuse, 'bytes_per_launch': nbytes,
+        'kernel': f"k_step_adv<{FORM_IDS[form]}, 1> (step + previous step's advection)" if s.fuse
+                  else f'k_step<{FORM_IDS[form]}, 1> (fused stream/collide)',
+        'achieved_gbs': nbytes / (step_ms * 1e-3) / 1e9,
         'cd': float(cd), 'cl': float(cl),
     }
     del s
@@ -132,8 +145,8 @@ def cpu_baseline(mask, target_s=12.0):
                                   'mlups_one_core': round(c1[1][1], 2), 'steps_one_core': c1[1][0]}}
 
 
-def load_traffic(form):
-    path = os.path.join(REPO, 'profiles', f'step_{form}_512x256x256_traffic.json')
+def load_traffic(form, fused):
+    path = os.path.join(REPO, 'profiles', f'step{"_adv" if fused else ""}_{form}_512x256x256_traffic.json')
     if os.path.exists(path):
         with open(path) as fh:
             return json.load(fh).get('hbm_bytes_per_launch')
@@ -201,8 +214,10 @@ def main():
                    'smag': SMAG, 'particles': PARTICLES, 'parallelism': 'single GPU'},
         'roofline': {'bound': 'hbm', 'achieved': round(r['achieved_gbs'], 1), 'peak': PEAK_HBM_GBS,
                      'unit': 'GB/s', 'frac': round(r['achieved_gbs'] / PEAK_HBM_GBS, 4),
-                     'traffic': load_traffic(main_form),
-                     'kernel': 'k_step (fused stream/collide)', 'bytes_per_cell': BYTES_PER_CELL[main_form],
+                     'traffic': load_traffic(main_form, r['fused_advect']),
+                     'kernel': r['kernel'], 'bytes_per_launch': r['bytes_per_launch'],
+                     'bytes_per_cell': BYTES_PER_CELL[main_form],
+                     'bytes_per_particle': BYTES_PER_PARTICLE if r['fused_advect'] else 0,
                      'kernel_ms': round(r['step_kernel_ms'], 4)},
         'cpu_baseline': cpu,
     }
@@ -210,11 +225,10 @@ def main():
         if form == main_form:
             continue
         line[form] = {'value': round(rr['mlups'], 1), 'ratio_vs_fp32': round(rr['mlups'] / r['mlups'], 3),
-                      'kernel_ms': round(rr['step_kernel_ms'], 4),
-                      'kernel_ratio_vs_fp32': round(r['step_kernel_ms'] / rr['step_kernel_ms'], 3),
+                      'kernel': rr['kernel'], 'kernel_ms': round(rr['step_kernel_ms'], 4),
                       'roofline_frac': round(rr['achieved_gbs'] / PEAK_HBM_GBS, 4),
-                      'achieved_gbs': round(rr['achieved_gbs'], 1), 'bytes_per_cell': BYTES_PER_CELL[form],
-                      'traffic': load_traffic(form)}
+                      'achieved_gbs': round(rr['achieved_gbs'], 1), 'bytes_per_launch': rr['bytes_per_launch'],
+                      'bytes_per_cell': BYTES_PER_CELL[form], 'traffic': load_traffic(form, rr['fused_advect'])}
     line['cd_cl'] = {f: [round(v['cd'], 4), round(v['cl'], 4)] for f, v in results.items()}
     if c4:
         line['config4_batch'] = {'value': round(c4['mlups'], 1), 'unit': 'MLUPS', 'tunnels': c4['tunnels'],

@@ -104,6 +104,16 @@ int wt_set_solid(const wt_grid* g, const uint8_t* solid_dev, void* stream);
  * in g->sums when the stream reaches the end of this call. Replaces the step
  * loop of Solver.step (lbm.py:442-447) over step_kernel (lbm.py:72-166). */
 int wt_step(const wt_grid* g, int cur, int nsteps, const wt_params* p, int flags, void* stream);
+/* One lattice step as wt_step(g, cur, 1, p, flags, stream), fused in the same
+ * launch with the particle advection of the previous step (slot storage, as
+ * wt_advect_slots; ids may be NULL when slot = id) sampled from mac_prev, the
+ * macroscopic field that step wrote. mac_prev must not be g->mac, which this
+ * step overwrites: callers alternate two mac buffers. Removes one kernel
+ * boundary per step (lbm.py:442-453 launches the step and advect_kernel
+ * separately); every result equals the two separate launches bit for bit. */
+int wt_step_advect(const wt_grid* g, int cur, const wt_params* p, int flags, const float* mac_prev, float* pos,
+                   float* spd, const uint32_t* ids, int64_t num_p, float dt, float uin, float max_age,
+                   uint32_t seed, const wt_seed_box* box, void* stream);
 /* Momentum-exchange force on f[cur] into g->sums[0..2] (FP64, deterministic).
  * Replaces forces.zero_() + force_kernel (lbm.py:479-481, 169-197). */
 int wt_forces(const wt_grid* g, int cur, void* stream);

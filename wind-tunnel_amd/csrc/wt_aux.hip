@@ -13,6 +13,7 @@
 
 #include "wt_device.h"
 #include "wt_host.h"
+#include "wt_particles.h"
 
 namespace wt {
 
@@ -23,116 +24,10 @@ char* error_buffer() {
 
 constexpr int ABLOCK = 256;
 
-__device__ __forceinline__ float clampf(float v, float a, float b) {
-  return (v < a) ? a : ((v > b) ? b : v);
-}
-__device__ __forceinline__ int clampi(int v, int a, int b) { return (v < a) ? a : ((v > b) ? b : v); }
-
-// lbm.py:201-203
-__device__ __forceinline__ uint32_t cidx(int x, int y, int z, int nx, int ny, int nz) {
-  return (uint32_t)clampi(x, 0, nx - 1) + (uint32_t)nx * ((uint32_t)clampi(y, 0, ny - 1) +
-                                                          (uint32_t)ny * (uint32_t)clampi(z, 0, nz - 1));
-}
-
-// Warp's lerp(a, b, t) = a*(1-t) + b*t, componentwise.
-__device__ __forceinline__ float4 lerp4(float4 a, float4 b, float t) {
-  const float s = 1.0f - t;
-  return make_float4(a.x * s + b.x * t, a.y * s + b.y * t, a.z * s + b.z * t, a.w * s + b.w * t);
-}
-
-// Trilinear sample of the macroscopic field, lbm.py:206-230. The 8 corner
-// values are kept between calls: the second RK2 sample of a particle usually
-// lies in the cell of the first, and then reuses them (same values, so the same
-// result) instead of gathering them again.
-struct Corners {
-  int ix, iy, iz;
-  float4 c[8];   // (x, y, z) offsets 000 100 010 110 001 101 011 111
-};
-
-__device__ __forceinline__ float4 sample_mac(const float4* __restrict__ mac, float px, float py, float pz, int nx,
-                                             int ny, int nz, Corners& k, bool have) {
-  const float qx = clampf(px, 0.0f, (float)nx - 1.001f);
-  const float qy = clampf(py, 0.0f, (float)ny - 1.001f);
-  const float qz = clampf(pz, 0.0f, (float)nz - 1.001f);
-  const float bx = floorf(qx), by = floorf(qy), bz = floorf(qz);
-  const float tx = qx - bx, ty = qy - by, tz = qz - bz;
-  const int ix = (int)bx, iy = (int)by, iz = (int)bz;
-  if (!(have && ix == k.ix && iy == k.iy && iz == k.iz)) {
-    k.ix = ix; k.iy = iy; k.iz = iz;
-#pragma unroll
-    for (int c = 0; c < 8; c++) k.c[c] = mac[cidx(ix + (c & 1), iy + ((c >> 1) & 1), iz + (c >> 2), nx, ny, nz)];
-  }
-  const float4 a = lerp4(lerp4(k.c[0], k.c[1], tx), lerp4(k.c[2], k.c[3], tx), ty);
-  const float4 b = lerp4(lerp4(k.c[4], k.c[5], tx), lerp4(k.c[6], k.c[7], tx), ty);
-  return lerp4(a, b, tz);
-}
-
-// Respawn stream: counter-based, PCG-RXS-M-XS output step applied to a state
-// derived from (seed, particle index). Same definition as the CPU oracle.
-__device__ __forceinline__ uint32_t pcg_hash(uint32_t v) {
-  const uint32_t s = v * 747796405u + 2891336453u;
-  const uint32_t w = ((s >> ((s >> 28u) + 4u)) ^ s) * 277803737u;
-  return (w >> 22u) ^ w;
-}
-__device__ __forceinline__ uint32_t rng_init(uint32_t seed, uint32_t i) {
-  return pcg_hash(seed ^ pcg_hash(i + 0x9e3779b9u));
-}
-__device__ __forceinline__ float rng_f(uint32_t& st) {
-  st = pcg_hash(st);
-  return (float)(st >> 8) * (1.0f / 16777216.0f);
-}
-
-struct AdvectArgs {
-  const float4* mac;
-  const uint8_t* cls;
-  const uint32_t* order;  // processing order (a permutation of particle slots) or null
-  const uint32_t* ids;    // particle id held in each slot (slot storage) or null (slot = id)
-  float4* pos;
-  float* spd;
-  uint32_t num_p;
-  int nx, ny, nz;
-  float dt, uin, max_age;
-  uint32_t seed;
-  wt_seed_box box;
-};
-
 // lbm.py:233-263
 __global__ __launch_bounds__(ABLOCK) void k_advect(AdvectArgs a) {
   const uint32_t j = blockIdx.x * ABLOCK + threadIdx.x;
-  if (j >= a.num_p) return;
-  // memory slot: lanes of a wave take spatial neighbours when an order is given
-  const uint32_t slot = a.order ? a.order[j] : j;
-  // particle id, which keys the respawn stream: the slot itself unless slots hold other ids
-  const uint32_t i = a.ids ? a.ids[slot] : slot;
-  const float4 p = a.pos[slot];
-  float qx = p.x, qy = p.y, qz = p.z;
-  Corners k;
-  const float4 m1 = sample_mac(a.mac, qx, qy, qz, a.nx, a.ny, a.nz, k, false);
-  const float h = 0.5f * a.dt;
-  const float4 m2 = sample_mac(a.mac, qx + h * m1.x, qy + h * m1.y, qz + h * m1.z, a.nx, a.ny, a.nz, k, true);
-  qx = qx + a.dt * m2.x;
-  qy = qy + a.dt * m2.y;
-  qz = qz + a.dt * m2.z;
-  float age = p.w + 1.0f;
-  float s = sqrtf(m2.x * m2.x + m2.y * m2.y + m2.z * m2.z) / a.uin;
-  bool dead = age > a.max_age || qx < 0.5f || qy < 0.5f || qz < 0.5f;
-  if (qx > (float)a.nx - 3.0f || qy > (float)a.ny - 1.5f || qz > (float)a.nz - 1.5f || !(s < 100.0f))
-    dead = true;
-  if (!dead) {
-    const uint32_t c = cidx((int)roundf(qx), (int)roundf(qy), (int)roundf(qz), a.nx, a.ny, a.nz);
-    if ((a.cls[c] & CLS_CODE) != 0) dead = true;
-  }
-  if (dead) {
-    uint32_t st = rng_init(a.seed, i);
-    const float r1 = rng_f(st), r2 = rng_f(st), r3 = rng_f(st);
-    qx = a.box.x + r1 * 2.0f;
-    qy = a.box.cy + (r2 * 2.0f - 1.0f) * a.box.hy;
-    qz = a.box.cz + (r3 * 2.0f - 1.0f) * a.box.hz;
-    age = 0.0f;
-    s = 1.0f;
-  }
-  a.pos[slot] = make_float4(qx, qy, qz, age);
-  a.spd[slot] = s;
+  if (j < a.num_p) advect_particle(a, j);
 }
 
 // Slot storage: out[j] = in[order[j]] for the particle arrays and their ids.

@@ -35,6 +35,7 @@
 
 #include "wt_device.h"
 #include "wt_host.h"
+#include "wt_particles.h"
 
 namespace wt {
 
@@ -578,6 +579,25 @@ __global__ __launch_bounds__(BLOCK) WT_STEP_ATTR void k_step(StepDesc d, int par
   run_tile<FORM, CPT>(d, parity, tile, flags);
 }
 
+// One lattice step plus the particle advection of the previous step from its
+// macroscopic field a.mac (a buffer the step does not write), in one launch.
+// The advection takes the first gridDim.x - step_blocks workgroups: its
+// latency-bound gathers then overlap the ramp-up of the HBM-bound tiles, which
+// follow in the tile order of k_step. (Measured at 33.5M cells and 2^20
+// particles: advection first costs ~7 us over the plain step, spread evenly
+// through the tiles ~14 us, after the tiles ~19 us, as much as its own launch.)
+template <int FORM, int CPT>
+__global__ __launch_bounds__(BLOCK) WT_STEP_ATTR void k_step_adv(StepDesc d, AdvectArgs a, int parity, int remap,
+                                                                 int flags, uint32_t step_blocks) {
+  const uint32_t nadv = gridDim.x - step_blocks;
+  if (blockIdx.x >= nadv) {
+    run_tile<FORM, CPT>(d, parity, xcd_tile(blockIdx.x - nadv, step_blocks, remap), flags);
+  } else {
+    const uint32_t j = blockIdx.x * BLOCK + threadIdx.x;
+    if (j < a.num_p) advect_particle(a, j);
+  }
+}
+
 template <int FORM, int CPT>
 __global__ __launch_bounds__(BLOCK) void k_step_batch(const StepDesc* __restrict__ ds, int nd,
                                                       int parity, int remap, int flags) {
@@ -885,6 +905,21 @@ static void launch_step(int form, const StepDesc& d, int parity, Shape sh, int f
 }
 
 template <int FORM, int CPT>
+static void launch_step_adv_t(const StepDesc& d, const AdvectArgs& a, int remap, int flags, hipStream_t s) {
+  const uint32_t ntile = (d.n + BLOCK * CPT - 1) / (BLOCK * CPT);
+  const uint32_t nadv = (a.num_p + BLOCK - 1) / BLOCK;
+  hipLaunchKernelGGL((k_step_adv<FORM, CPT>), dim3(ntile + nadv), dim3(BLOCK), 0, s, d, a, 0, remap, flags, ntile);
+}
+template <int FORM>
+static void launch_step_adv_f(const StepDesc& d, const AdvectArgs& a, Shape sh, int flags, hipStream_t s) {
+  switch (sh.cpt) {
+    case 2: launch_step_adv_t<FORM, 2>(d, a, sh.remap, flags, s); break;
+    case 4: launch_step_adv_t<FORM, 4>(d, a, sh.remap, flags, s); break;
+    default: launch_step_adv_t<FORM, 1>(d, a, sh.remap, flags, s); break;
+  }
+}
+
+template <int FORM, int CPT>
 static void launch_batch_t(const StepDesc* dd, int nd, uint32_t ntile, int parity, int remap,
                            int flags, hipStream_t s) {
   hipLaunchKernelGGL((k_step_batch<FORM, CPT>), dim3(ntile), dim3(BLOCK), 0, s, dd, nd, parity, remap,
@@ -1002,6 +1037,36 @@ int wt_step(const wt_grid* g, int cur, int nsteps, const wt_params* p, int flags
     return reduce_one(g, s);
   }
   return check_launch("wt_step");
+}
+
+int wt_step_advect(const wt_grid* g, int cur, const wt_params* p, int flags, const float* mac_prev, float* pos,
+                   float* spd, const uint32_t* ids, int64_t num_p, float dt, float uin, float max_age,
+                   uint32_t seed, const wt_seed_box* box, void* stream) {
+  if (int e = check_grid(g, true)) return e;
+  if (!p) return fail(-1, "null params");
+  if ((flags & (WT_FLAG_FORCES | WT_FLAG_MASS)) && (!g->partials || !g->sums))
+    return fail(-1, "reduction flags need partials and sums buffers");
+  if (num_p < 0 || num_p >= ((int64_t)1 << 31)) return fail(-1, "bad particle count %lld", (long long)num_p);
+  if (num_p > 0 && (!pos || !spd || !box || !mac_prev)) return fail(-1, "null particle buffer, field or seed box");
+  if (num_p > 0 && (const void*)mac_prev == (const void*)g->mac)
+    return fail(-1, "the advected field must not be the grid's mac buffer (the step writes it)");
+  hipStream_t s = (hipStream_t)stream;
+  Shape sh = g_shape[g->form];
+  sh.remap = (sh.remap & 1) | (((sh.remap & 2) && (cur & 1)) ? 2 : 0);   // as step k = 0 of wt_step
+  StepDesc d = make_desc(g, cur, p, 0);
+  if (g_shape[g->form].remap & 4) d.global_addr = 1;
+  const wt_seed_box none{0.f, 0.f, 0.f, 0.f, 0.f};
+  const AdvectArgs a{reinterpret_cast<const float4*>(mac_prev), g->cls, nullptr, ids, reinterpret_cast<float4*>(pos),
+                     spd, (uint32_t)num_p, g->nx, g->ny, g->nz, dt, uin, max_age, seed, box ? *box : none};
+  const int f = flags & (WT_FLAG_FORCES | WT_FLAG_MASS);
+  switch (g->form) {
+    case WT_FP32: launch_step_adv_f<WT_FP32>(d, a, sh, f, s); break;
+    case WT_FP16: launch_step_adv_f<WT_FP16>(d, a, sh, f, s); break;
+    case WT_BF16: launch_step_adv_f<WT_BF16>(d, a, sh, f, s); break;
+    case WT_FP64: launch_step_adv_f<WT_FP64>(d, a, sh, f, s); break;
+  }
+  if (int e = check_launch("wt_step_advect")) return e;
+  return f ? reduce_one(g, s) : 0;
 }
 
 int wt_forces(const wt_grid* g, int cur, void* stream) {

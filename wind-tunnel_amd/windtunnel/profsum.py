@@ -8,7 +8,9 @@
         same counter reads for wt_stream_copy of B bytes (reads B, writes B)
 
 Prints markdown; with --json-dir also writes step_<form>_<dims>_traffic.json
-files that bench.py reports as roofline.traffic.
+(k_step) and step_adv_<form>_<dims>_traffic.json (k_step_adv, the step fused
+with the advection of --particles particles) files that bench.py reports as
+roofline.traffic.
 """
 import argparse
 import collections
@@ -20,6 +22,7 @@ import re
 import sys
 
 BYTES_PER_CELL = {0: 169, 1: 93, 2: 93, 3: 321}
+BYTES_PER_PARTICLE = 40   # id 4, position 16 read + 16 written, speed 4 written
 FORM_NAMES = {0: 'fp32', 1: 'fp16', 2: 'bf16', 3: 'fp64'}
 
 
@@ -58,7 +61,7 @@ def _per_kernel(rows, counter):
     return acc
 
 
-def traffic(fetch_dir, write_dir, copy_bytes, json_dir=None, cells=None):
+def traffic(fetch_dir, write_dir, copy_bytes, json_dir=None, cells=None, particles=0):
     fr = _per_kernel(_rows(fetch_dir, 'counter_collection.csv'), 'FETCH_SIZE')
     wr = _per_kernel(_rows(write_dir, 'counter_collection.csv'), 'WRITE_SIZE')
     cf = [v for (k, _), vs in fr.items() if 'k_stream_copy' in k for v in vs]
@@ -71,7 +74,7 @@ def traffic(fetch_dir, write_dir, copy_bytes, json_dir=None, cells=None):
     w_scale = copy_bytes / cw[len(cw) // 2]
     lines = [f'Scaling: wt_stream_copy of {copy_bytes} B reads FETCH_SIZE {cf[len(cf) // 2]:.0f} and '
              f'WRITE_SIZE {cw[len(cw) // 2]:.0f} -> {f_scale:.2f} / {w_scale:.2f} bytes per counted unit', '',
-             '| kernel | grid | dispatches | fetch (MB) | write (MB) | total (MB) | B/cell | algorithmic B/cell |',
+             '| kernel | grid | dispatches | fetch (MB) | write (MB) | total (MB) | algorithmic (MB) | ratio |',
              '|---|---:|---:|---:|---:|---:|---:|---:|']
     out = {}
     for (name, grid), vs in sorted(fr.items()):
@@ -82,19 +85,23 @@ def traffic(fetch_dir, write_dir, copy_bytes, json_dir=None, cells=None):
             continue
         fb = sum(vs) / len(vs) * f_scale
         wb = sum(ws) / len(ws) * w_scale
-        m = re.search(r'k_step<(\d)', name)
-        form = int(m.group(1)) if m else 0
-        n = cells or (grid and grid)  # grid counts threads = cells for cells-per-thread 1
-        per_cell = (fb + wb) / n if n else float('nan')
+        m = re.search(r'k_step(_adv)?<(\d)', name)
+        if not m or 'batch' in name:
+            continue
+        adv, form = bool(m.group(1)), int(m.group(2))
+        algo = BYTES_PER_CELL[form] * cells + (BYTES_PER_PARTICLE * particles if adv else 0)
         lines.append(f'| {short(name)} | {grid} | {len(vs)} | {fb / 1e6:.1f} | {wb / 1e6:.1f} | {(fb + wb) / 1e6:.1f} | '
-                     f'{per_cell:.1f} | {BYTES_PER_CELL[form]} |')
-        out[FORM_NAMES[form]] = {'kernel': short(name), 'grid': grid, 'fetch_bytes': fb, 'write_bytes': wb,
-                                 'hbm_bytes_per_launch': fb + wb, 'bytes_per_cell': per_cell,
-                                 'algorithmic_bytes_per_cell': BYTES_PER_CELL[form],
-                                 'scale_fetch': f_scale, 'scale_write': w_scale}
+                     f'{algo / 1e6:.1f} | {(fb + wb) / algo:.4f} |')
+        key = ('step_adv_' if adv else 'step_') + FORM_NAMES[form]
+        out[key] = {'kernel': short(name), 'grid': grid, 'cells': cells, 'particles': particles if adv else 0,
+                    'fetch_bytes': fb, 'write_bytes': wb, 'hbm_bytes_per_launch': fb + wb,
+                    'bytes_per_cell': (fb + wb) / cells, 'algorithmic_bytes_per_launch': algo,
+                    'algorithmic_bytes_per_cell': BYTES_PER_CELL[form],
+                    'algorithmic_bytes_per_particle': BYTES_PER_PARTICLE if adv else 0,
+                    'scale_fetch': f_scale, 'scale_write': w_scale}
     if json_dir:
-        for form, v in out.items():
-            with open(os.path.join(json_dir, f'step_{form}_512x256x256_traffic.json'), 'w') as fh:
+        for key, v in out.items():
+            with open(os.path.join(json_dir, f'{key}_512x256x256_traffic.json'), 'w') as fh:
                 json.dump(v, fh, indent=1)
     return '\n'.join(lines)
 
@@ -111,11 +118,12 @@ def main(argv=None):
     b.add_argument('--copy-bytes', type=int, required=True)
     b.add_argument('--json-dir', default=None)
     b.add_argument('--cells', type=int, default=512 * 256 * 256)
+    b.add_argument('--particles', type=int, default=0)
     args = ap.parse_args(argv)
     if args.cmd == 'stats':
         print(stats(args.dir, args.top))
     else:
-        print(traffic(args.fetch_dir, args.write_dir, args.copy_bytes, args.json_dir, args.cells))
+        print(traffic(args.fetch_dir, args.write_dir, args.copy_bytes, args.json_dir, args.cells, args.particles))
     return 0
 
 

@@ -93,6 +93,17 @@ class Tunnel:
             dev = torch.from_numpy(mask).pin_memory().to(self.device, non_blocking=True)
             check(self.lib.wt_set_solid(self.gp, _ptr(dev), stream_handle(s)), 'wt_set_solid')
 
+    def set_mac(self, mac):
+        """Point later launches at another (n, 4) float32 macroscopic buffer. The
+        step kernel only writes mac (face cells read the shell snapshot), so a
+        caller may alternate two buffers and read one while a step writes the other."""
+        same_dev = mac.device.type == self.device.type and (self.device.index is None or
+                                                            mac.device.index == self.device.index)
+        if tuple(mac.shape) != (self.n, 4) or mac.dtype != torch.float32 or not same_dev:
+            raise ValueError('mac buffer must be float32 (n, 4) on the tunnel device')
+        self.mac = mac
+        self.grid.mac = mac.data_ptr()
+
     # ---- hot path ----------------------------------------------------------
     def step(self, nsteps, uin, tau, smag, flags=0, stream=None):
         if self.shape is not None:
@@ -101,6 +112,21 @@ class Tunnel:
         check(self.lib.wt_step(self.gp, self.cur, int(nsteps), ctypes.byref(p), int(flags),
                                stream_handle(stream)), 'wt_step')
         self.cur ^= int(nsteps) & 1
+
+    def step_advect(self, uin, tau, smag, flags, mac_prev, pos, spd, ids, dt, max_age, seed, box, stream=None):
+        """One step fused with the advection (slot storage) of the particles of the
+        previous step from mac_prev, the field that step wrote (not self.mac)."""
+        if self.shape is not None:
+            apply_shape(self.form, self.shape)
+        p = Params(float(uin), float(tau), float(smag))
+        sb = SeedBox(*(float(v) for v in box))
+        n = int(pos.shape[0]) if pos is not None else 0
+        check(self.lib.wt_step_advect(self.gp, self.cur, ctypes.byref(p), int(flags), _ptr(mac_prev),
+                                      _ptr(pos) if n else None, _ptr(spd) if n else None,
+                                      None if ids is None else _ptr(ids), n, float(dt), float(uin), float(max_age),
+                                      int(seed) & 0xffffffff, ctypes.byref(sb), stream_handle(stream)),
+              'wt_step_advect')
+        self.cur ^= 1
 
     def forces(self, stream=None):
         """Momentum-exchange force on f[cur] into self.sums[0:3] (device)."""

@@ -5,7 +5,9 @@ a 512 x 256 x 256 = 33.5M-cell tunnel, car-like body = seeded union of 12
 ellipsoids on a moving ground belt, uin 0.05, tau 0.51, Cs 0.16, 2^20 particles
 seeded on the inlet plane and advected every step, three mid-span slice planes
 (speed, pressure, vorticity) and the force extracted every 10 steps. A "step"
-is one lattice update of the whole tunnel plus the particle advection.
+is one lattice update of the whole tunnel plus the particle advection. By
+default each storage form runs 20 warm-up and 2000 timed steps; the batch of
+48 tunnels runs 1000 steps and the serving mix a 30 s window.
 
 value = MLUPS with FP32 populations (all nx*ny*nz cells per step, as
 server/app.py:180 counts them); the same workload with FP16-deviation storage
@@ -160,11 +162,11 @@ def load_traffic(form, fused):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument('--gpus', type=int, default=1)
-    ap.add_argument('--steps', type=int, default=1000)
+    ap.add_argument('--steps', type=int, default=2000)
     ap.add_argument('--warmup', type=int, default=20)
     ap.add_argument('--forms', default='fp32,fp16')
     ap.add_argument('--no-cpu', action='store_true')
-    ap.add_argument('--c4-steps', type=int, default=500)
+    ap.add_argument('--c4-steps', type=int, default=1000)
     ap.add_argument('--serve-seconds', type=float, default=30.0)
     args = ap.parse_args()
     if args.gpus != 1:

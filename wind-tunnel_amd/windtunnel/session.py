@@ -132,7 +132,7 @@ class Session:
             if events is not None:
                 events[1].record(main)
             if self.num_p:
-                self.advect(main)
+                self._advect(main)
         if frame:
             self.emit_frame(main)
         elif want_force:
@@ -148,7 +148,7 @@ class Session:
             self._spare = (self.pos, self.spd, self.ids)
             self.pos, self.spd, self.ids = pos2, spd2, ids2
 
-    def advect(self, stream=None):
+    def _advect(self, stream=None):
         """Particle advection with dt = 1 on slot storage re-sorted by cell every sort_every steps."""
         self._sort_if_due(stream)
         self.seed += 1
@@ -159,7 +159,7 @@ class Session:
     def flush(self, stream=None):
         """Advect the particles of the last step if a fused launch has not yet."""
         if self.fuse and self.pending:
-            self.advect(stream)
+            self._advect(stream)
             self.pending = False
 
     def particles_by_id(self):

@@ -303,80 +303,11 @@ __device__ __forceinline__ R face_feq(int i, R br, R bux, R buy, R buz, R bsq) {
   return (R)wd(i) * br * ((R)1 + bc + (R)0.5 * bc * bc - bsq);
 }
 
-// ---- one cell, general path (any mix of solid, face and near-wall lanes) ------------
-template <int FORM>
-__device__ __forceinline__ void step_cell_general(const StepDesc& d, const typename Form<FORM>::elem* fin,
-                                                  typename Form<FORM>::elem* fout,
-                                                  const Shell<typename Form<FORM>::real>& shin,
-                                                  const Shell<typename Form<FORM>::real>& shout,
-                                                  uint32_t idx, uint32_t pc, uint8_t c, int x, int y, int z,
-                                                  bool face, int flags, Acc& acc) {
-  typedef Form<FORM> S;
-  typedef typename S::real R;
-  typedef typename S::elem E;
-  const int32_t nx = d.nx, ny = d.ny;
-  const int32_t nxy = nx * ny;
-  const R zero = (R)0, one = (R)1;
-
-  if ((c & CLS_CODE) != 0) {  // solid cell, lbm.py:81-83
-    st_mac(d.mac + idx, 0.f, 0.f, 0.f, 1.f);
-    shell_write<R>(shout, x, y, z, zero, zero, zero, one);
-    return;
-  }
-  if (face) {  // equilibrium boundaries, lbm.py:84-109
-    R br, bux, buy, buz;
-    face_state<R>(d, shin, x, y, z, br, bux, buy, buz);
-    const R bsq = (R)1.5 * (bux * bux + buy * buy + buz * buz);
-#pragma unroll
-    for (int i = 0; i < Q; i++)
-      st_stream(fout + (pc + (uint32_t)ppos(i) * (uint32_t)nx), S::store(face_feq<R>(i, br, bux, buy, buz, bsq), i));
-    st_mac(d.mac + idx, (float)bux, (float)buy, (float)buz, (float)br);
-    if (flags & WT_FLAG_MASS) acc.m += (double)br;
-    return;
-  }
-
-  // pull streaming with bounce-back and the moving belt, lbm.py:110-124
-  Links lk{0u, 0u};
-  if (c & CLS_NEAR) lk = gather_links(d.cls, idx, nx, nxy);
-  R f[Q];
-#pragma unroll
-  for (int i = 0; i < Q; i++) {
-    if (i == 0) {
-      f[0] = S::load(ld_stream(fin + pc), 0);
-    } else {
-      const bool b = (lk.bb >> i) & 1u;
-      const uint32_t a = pc + (b ? (uint32_t)ppos(opp(i)) * (uint32_t)nx : pshift(i, nx, nxy));
-      f[i] = S::load(ld_stream(fin + a), i);
-    }
-  }
-  if (lk.belt) {
-    const R uin = (R)d.uin;
-#pragma unroll
-    for (int i = 1; i < Q; i++)
-      if (cx(i) != 0 && ((lk.belt >> i) & 1u)) f[i] = f[i] + (R)(6.0 * wd(i) * cx(i)) * uin;
-  }
-  R out[Q], ux, uy, uz, rho;
-  collide<R>(f, x, nx, (R)d.tau, (R)d.smag, out, ux, uy, uz, rho);
-  E st[Q];
-#pragma unroll
-  for (int i = 0; i < Q; i++) {
-    st[i] = S::store(out[i], i);
-    st_stream(fout + (pc + (uint32_t)ppos(i) * (uint32_t)nx), st[i]);
-  }
-  st_mac(d.mac + idx, (float)ux, (float)uy, (float)uz, (float)rho);
-  shell_write<R>(shout, x, y, z, ux, uy, uz, rho);
-  if (flags & WT_FLAG_MASS) acc.m += (double)rho;
-  if ((flags & WT_FLAG_FORCES) && (c & CLS_FORCE)) add_force<FORM>(d.cls, idx, nx, nxy, st, acc);
-}
-
-// ---- one cell, fast path: the whole wave is interior fluid away from walls -----------
-// Every lane pulls from its 18 neighbours. With the row-blocked layout the load
-// of population i is the lane's own element offset (one vector register shared
-// by all 38 loads and stores) plus the wave-uniform pshift(i) in a scalar
-// register. For 2- and 4-byte storage the populations are addressed as one raw
-// buffer (hardware range check on) whose base sits `bias` bytes below the
-// allocation so that every scalar offset is non-negative; buffers beyond 32-bit
-// byte offsets use 64-bit global addresses instead.
+// ---- population accesses ------------------------------------------------------------
+// For 2- and 4-byte storage the populations are addressed as one raw buffer
+// (hardware range check on) with 32-bit offsets; its base sits `bias` bytes
+// below the allocation so that every pull offset is non-negative. Buffers
+// beyond 32-bit byte offsets (and FP64) use 64-bit global addresses instead.
 constexpr int BUF_GFX9_WORD3 = 0x00020000;  // raw buffer, gfx9-family resource word 3
 // Directions in ascending memory order of their upstream rows (z-1 rows first,
 // then y-1, the own row, y+1, z+1; x+1 before x-1 within a row): issuing the 19
@@ -410,6 +341,120 @@ template <> struct RawIO<uint16_t> {
   }
 };
 
+// ---- one cell, general path (any mix of solid, face and near-wall lanes) ------------
+template <int FORM>
+__device__ __forceinline__ void step_cell_general(const StepDesc& d, const typename Form<FORM>::elem* fin,
+                                                  typename Form<FORM>::elem* fout,
+                                                  const Shell<typename Form<FORM>::real>& shin,
+                                                  const Shell<typename Form<FORM>::real>& shout,
+                                                  uint32_t idx, uint32_t pc, uint8_t c, int x, int y, int z,
+                                                  bool face, int flags, Acc& acc) {
+  typedef Form<FORM> S;
+  typedef typename S::real R;
+  typedef typename S::elem E;
+  const int32_t nx = d.nx, ny = d.ny;
+  const int32_t nxy = nx * ny;
+  const R zero = (R)0, one = (R)1;
+
+  if ((c & CLS_CODE) != 0) {  // solid cell, lbm.py:81-83
+    st_mac(d.mac + idx, 0.f, 0.f, 0.f, 1.f);
+    shell_write<R>(shout, x, y, z, zero, zero, zero, one);
+    return;
+  }
+  constexpr uint32_t ES = (uint32_t)sizeof(E);
+  const uint32_t n = d.n;
+  const uint32_t bo = pc * ES;
+  // stores of the 19 populations of this cell (own row block, plane offsets)
+  auto store_all = [&](const E (&v)[Q]) {
+    auto global_stores = [&]() {
+#pragma unroll
+      for (int i = 0; i < Q; i++) st_stream(fout + (pc + (uint32_t)ppos(i) * (uint32_t)nx), v[i]);
+    };
+    if constexpr (sizeof(E) <= 4) {
+      if (!d.global_addr) {
+        const __amdgpu_buffer_rsrc_t rout =
+            __builtin_amdgcn_make_buffer_rsrc((void*)fout, (short)0, (int)((uint32_t)Q * n * ES), BUF_GFX9_WORD3);
+#pragma unroll
+        for (int i = 0; i < Q; i++) RawIO<E>::st(rout, bo, (uint32_t)ppos(i) * (uint32_t)nx * ES, v[i]);
+      } else {
+        global_stores();
+      }
+    } else {
+      global_stores();
+    }
+  };
+  if (face) {  // equilibrium boundaries, lbm.py:84-109
+    R br, bux, buy, buz;
+    face_state<R>(d, shin, x, y, z, br, bux, buy, buz);
+    const R bsq = (R)1.5 * (bux * bux + buy * buy + buz * buz);
+    E v[Q];
+#pragma unroll
+    for (int i = 0; i < Q; i++) v[i] = S::store(face_feq<R>(i, br, bux, buy, buz, bsq), i);
+    store_all(v);
+    st_mac(d.mac + idx, (float)bux, (float)buy, (float)buz, (float)br);
+    if (flags & WT_FLAG_MASS) acc.m += (double)br;
+    return;
+  }
+
+  // pull streaming with bounce-back and the moving belt, lbm.py:110-124:
+  // population i comes from element pc + pshift(i) (the upstream neighbour), or
+  // from the cell's own opposite plane pc + ppos(opp(i)) * nx when that
+  // neighbour is a wall; the loads go out in ascending upstream-row order
+  Links lk{0u, 0u};
+  if (c & CLS_NEAR) lk = gather_links(d.cls, idx, nx, nxy);
+  R f[Q];
+  auto global_loads = [&]() {
+#pragma unroll
+    for (int k = 0; k < Q; k++) {
+      const int i = LOAD_ORDER[k];
+      const bool b = (lk.bb >> i) & 1u;
+      const uint32_t a = pc + (b ? (uint32_t)ppos(opp(i)) * (uint32_t)nx : pshift(i, nx, nxy));
+      f[i] = S::load(ld_stream(fin + a), i);
+    }
+  };
+  if constexpr (sizeof(E) <= 4) {
+    if (!d.global_addr) {
+      // per-lane offset bo + bias + (element shift) * ES: non-negative, and below
+      // Q * n * ES + bias for every element of the array (no 32-bit wrap)
+      const uint32_t bias = (uint32_t)Q * ((uint32_t)nxy + (uint32_t)nx) * ES;
+      const __amdgpu_buffer_rsrc_t rin = __builtin_amdgcn_make_buffer_rsrc(
+          (void*)((const char*)fin - bias), (short)0, (int)((uint32_t)Q * n * ES + bias), BUF_GFX9_WORD3);
+#pragma unroll
+      for (int k = 0; k < Q; k++) {
+        const int i = LOAD_ORDER[k];
+        const bool b = (lk.bb >> i) & 1u;
+        const uint32_t sel = (b ? (uint32_t)ppos(opp(i)) * (uint32_t)nx * ES : pshift(i, nx, nxy) * ES) + bias;
+        f[i] = S::load(RawIO<E>::ld(rin, bo + sel, 0u), i);
+      }
+    } else {
+      global_loads();
+    }
+  } else {
+    global_loads();
+  }
+  if (lk.belt) {
+    const R uin = (R)d.uin;
+#pragma unroll
+    for (int i = 1; i < Q; i++)
+      if (cx(i) != 0 && ((lk.belt >> i) & 1u)) f[i] = f[i] + (R)(6.0 * wd(i) * cx(i)) * uin;
+  }
+  R out[Q], ux, uy, uz, rho;
+  collide<R>(f, x, nx, (R)d.tau, (R)d.smag, out, ux, uy, uz, rho);
+  E st[Q];
+#pragma unroll
+  for (int i = 0; i < Q; i++) st[i] = S::store(out[i], i);
+  store_all(st);
+  st_mac(d.mac + idx, (float)ux, (float)uy, (float)uz, (float)rho);
+  shell_write<R>(shout, x, y, z, ux, uy, uz, rho);
+  if (flags & WT_FLAG_MASS) acc.m += (double)rho;
+  if ((flags & WT_FLAG_FORCES) && (c & CLS_FORCE)) add_force<FORM>(d.cls, idx, nx, nxy, st, acc);
+}
+
+// ---- one cell, fast path: the whole wave is interior fluid away from walls -----------
+// Every lane pulls from its 18 neighbours. With the row-blocked layout the load
+// of population i is the lane's own element offset (one vector register shared
+// by all 38 loads and stores) plus the wave-uniform pshift(i) in a scalar
+// register (raw-buffer offsets, as above), with no per-lane address arithmetic.
 template <int FORM>
 __device__ __forceinline__ void step_cell_fast(const StepDesc& d, const typename Form<FORM>::elem* fin,
                                                typename Form<FORM>::elem* fout,

@@ -5,6 +5,8 @@ config4: 48 independent tunnels of uneven size drawn by seed from
 ellipsoid obstacle, inlet speed U(0.03, 0.08), tau U(0.51, 0.60), FP32; all of
 them advance in ONE batched launch per step, with forces every step fetched
 in one copy.
+
+    python -m windtunnel.workloads [--steps 1000] [--warmup 20] [--form fp32] [--no-graphs]
 """
 import time
 
@@ -79,3 +81,21 @@ def run_config4(steps=1000, warmup=20, form='fp32', graphs=False, seed=4):
     return {'tunnels': len(tunnels), 'cells': cells, 'steps': done, 'graphs': graphs, 'wall_s': wall,
             'gpu_ms': e0.elapsed_time(e1), 'mlups': cells * done / wall / 1e6,
             'ms_per_step': wall * 1e3 / done}
+
+
+def main(argv=None):
+    import argparse
+    import json
+    ap = argparse.ArgumentParser(description='config-4 batch: 48 uneven tunnels, one batched launch per step')
+    ap.add_argument('--steps', type=int, default=1000)
+    ap.add_argument('--warmup', type=int, default=20)
+    ap.add_argument('--form', default='fp32')
+    ap.add_argument('--no-graphs', action='store_true')
+    args = ap.parse_args(argv)
+    print(json.dumps(run_config4(args.steps, args.warmup, args.form, graphs=not args.no_graphs)), flush=True)
+    return 0
+
+
+if __name__ == '__main__':
+    import sys
+    sys.exit(main())

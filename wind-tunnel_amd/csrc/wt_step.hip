@@ -422,9 +422,11 @@ __device__ __forceinline__ void step_cell_general(const StepDesc& d, const typen
 #pragma unroll
       for (int k = 0; k < Q; k++) {
         const int i = LOAD_ORDER[k];
-        const bool b = (lk.bb >> i) & 1u;
-        const uint32_t sel = (b ? (uint32_t)ppos(opp(i)) * (uint32_t)nx * ES : pshift(i, nx, nxy) * ES) + bias;
-        f[i] = S::load(RawIO<E>::ld(rin, bo + sel, 0u), i);
+        // branch-free select between the two wave-uniform offsets
+        const uint32_t pull = pshift(i, nx, nxy) * ES + bias;
+        const uint32_t wall = (uint32_t)ppos(opp(i)) * (uint32_t)nx * ES + bias;
+        const uint32_t m = 0u - ((lk.bb >> i) & 1u);
+        f[i] = S::load(RawIO<E>::ld(rin, bo + (pull + (m & (wall - pull))), 0u), i);
       }
     } else {
       global_loads();

@@ -39,6 +39,10 @@ def _ptr(t):
     return ctypes.c_void_p(t.data_ptr())
 
 
+F32, I32 = (torch.float32,), (torch.int32,)
+SLICE_AXES = {2: ('nx', 'ny'), 1: ('nx', 'nz'), 0: ('nz', 'ny')}   # slice_dims, lbm.py:455-456
+
+
 class Tunnel:
     """One tunnel resident in HBM."""
 
@@ -66,6 +70,37 @@ class Tunnel:
         self.cur = 0
         # launch shape of this tunnel's size class (None: keep the library's current shape)
         self.shape = autotune.shape_for(self.form_name, self.n)
+
+    def _arg(self, t, name, dtypes, shape=None, min_numel=None):
+        """Pointer to tensor t after checking it: the kernels take raw pointers and
+        sizes and cannot tell a wrong buffer from a right one, so a tensor on
+        another device, of another dtype, not contiguous, of another shape or
+        too small raises ValueError here instead of reaching the GPU."""
+        if not isinstance(t, torch.Tensor):
+            raise ValueError(f'{name} must be a torch tensor, got {type(t).__name__}')
+        if t.device.type != self.device.type or (self.device.index is not None and
+                                                 t.device.index != self.device.index):
+            raise ValueError(f'{name} is on {t.device}, the tunnel on {self.device}')
+        if t.dtype not in dtypes:
+            raise ValueError(f'{name} must be {"/".join(str(d) for d in dtypes)}, got {t.dtype}')
+        if not t.is_contiguous():
+            raise ValueError(f'{name} must be contiguous')
+        if shape is not None and tuple(t.shape) != tuple(shape):
+            raise ValueError(f'{name} must have shape {tuple(shape)}, got {tuple(t.shape)}')
+        if min_numel is not None and t.numel() < min_numel:
+            raise ValueError(f'{name} holds {t.numel()} elements, needs {min_numel}')
+        return _ptr(t)
+
+    def _particles(self, pos, spd, **ints):
+        """Pointers to pos (n, 4) float32, spd (n,) float32 and the named (n,) int32
+        index arrays (None passes through as a null pointer); returns (n, ptrs)."""
+        if not isinstance(pos, torch.Tensor) or pos.dim() != 2:
+            raise ValueError('pos must be an (n, 4) float32 tensor')
+        n = int(pos.shape[0])
+        ptrs = [self._arg(pos, 'pos', F32, (n, 4)), self._arg(spd, 'spd', F32, (n,))]
+        for name, t in ints.items():
+            ptrs.append(None if t is None else self._arg(t, name, I32, (n,)))
+        return n, ptrs
 
     @property
     def form_name(self):
@@ -97,10 +132,7 @@ class Tunnel:
         """Point later launches at another (n, 4) float32 macroscopic buffer. The
         step kernel only writes mac (face cells read the shell snapshot), so a
         caller may alternate two buffers and read one while a step writes the other."""
-        same_dev = mac.device.type == self.device.type and (self.device.index is None or
-                                                            mac.device.index == self.device.index)
-        if tuple(mac.shape) != (self.n, 4) or mac.dtype != torch.float32 or not same_dev:
-            raise ValueError('mac buffer must be float32 (n, 4) on the tunnel device')
+        self._arg(mac, 'mac', F32, (self.n, 4))
         self.mac = mac
         self.grid.mac = mac.data_ptr()
 
@@ -120,12 +152,11 @@ class Tunnel:
             apply_shape(self.form, self.shape)
         p = Params(float(uin), float(tau), float(smag))
         sb = SeedBox(*(float(v) for v in box))
-        n = int(pos.shape[0]) if pos is not None else 0
-        check(self.lib.wt_step_advect(self.gp, self.cur, ctypes.byref(p), int(flags), _ptr(mac_prev),
-                                      _ptr(pos) if n else None, _ptr(spd) if n else None,
-                                      None if ids is None else _ptr(ids), n, float(dt), float(uin), float(max_age),
-                                      int(seed) & 0xffffffff, ctypes.byref(sb), stream_handle(stream)),
-              'wt_step_advect')
+        mp = self._arg(mac_prev, 'mac_prev', F32, (self.n, 4))
+        n, (pp, sp, ip) = self._particles(pos, spd, ids=ids)
+        check(self.lib.wt_step_advect(self.gp, self.cur, ctypes.byref(p), int(flags), mp, pp, sp, ip, n, float(dt),
+                                      float(uin), float(max_age), int(seed) & 0xffffffff, ctypes.byref(sb),
+                                      stream_handle(stream)), 'wt_step_advect')
         self.cur ^= 1
 
     def forces(self, stream=None):
@@ -192,53 +223,83 @@ class Tunnel:
     # ---- particles, slices, surface -------------------------------------------
     def advect(self, pos, spd, dt, uin, max_age, seed, box, stream=None):
         sb = SeedBox(*(float(v) for v in box))
-        check(self.lib.wt_advect(self.gp, _ptr(pos), _ptr(spd), int(pos.shape[0]), float(dt), float(uin),
-                                 float(max_age), int(seed) & 0xffffffff, ctypes.byref(sb),
-                                 stream_handle(stream)), 'wt_advect')
+        n, (pp, sp) = self._particles(pos, spd)
+        check(self.lib.wt_advect(self.gp, pp, sp, n, float(dt), float(uin), float(max_age), int(seed) & 0xffffffff,
+                                 ctypes.byref(sb), stream_handle(stream)), 'wt_advect')
 
     def advect_ordered(self, pos, spd, order, dt, uin, max_age, seed, box, stream=None):
         sb = SeedBox(*(float(v) for v in box))
-        check(self.lib.wt_advect_ordered(self.gp, _ptr(pos), _ptr(spd), _ptr(order), int(pos.shape[0]), float(dt),
-                                         float(uin), float(max_age), int(seed) & 0xffffffff, ctypes.byref(sb),
-                                         stream_handle(stream)), 'wt_advect_ordered')
+        n, (pp, sp, op) = self._particles(pos, spd, order=order)
+        check(self.lib.wt_advect_ordered(self.gp, pp, sp, op, n, float(dt), float(uin), float(max_age),
+                                         int(seed) & 0xffffffff, ctypes.byref(sb), stream_handle(stream)),
+              'wt_advect_ordered')
 
     def advect_slots(self, pos, spd, ids, dt, uin, max_age, seed, box, stream=None):
         """Advection of slot storage: slot j holds particle ids[j] (int32 tensor)."""
         sb = SeedBox(*(float(v) for v in box))
-        check(self.lib.wt_advect_slots(self.gp, _ptr(pos), _ptr(spd), _ptr(ids), int(pos.shape[0]), float(dt),
-                                       float(uin), float(max_age), int(seed) & 0xffffffff, ctypes.byref(sb),
-                                       stream_handle(stream)), 'wt_advect_slots')
+        n, (pp, sp, ip) = self._particles(pos, spd, ids=ids)
+        check(self.lib.wt_advect_slots(self.gp, pp, sp, ip, n, float(dt), float(uin), float(max_age),
+                                       int(seed) & 0xffffffff, ctypes.byref(sb), stream_handle(stream)),
+              'wt_advect_slots')
 
     def particle_gather(self, pos, spd, ids, order, pos_out, spd_out, ids_out, stream=None):
         """pos_out[j] = pos[order[j]] (and spd, ids; ids None = identity)."""
-        check(self.lib.wt_particle_gather(_ptr(pos), _ptr(spd), None if ids is None else _ptr(ids), _ptr(order),
-                                          int(pos.shape[0]), _ptr(pos_out), _ptr(spd_out), _ptr(ids_out),
-                                          stream_handle(stream)), 'wt_particle_gather')
+        n, (pp, sp, ip, op) = self._particles(pos, spd, ids=ids, order=order)
+        _, (po, so, io) = self._particles(pos_out, spd_out, ids_out=ids_out)
+        if int(pos_out.shape[0]) != n or io is None:
+            raise ValueError('particle_gather outputs must match the inputs (ids_out included)')
+        check(self.lib.wt_particle_gather(pp, sp, ip, op, n, po, so, io, stream_handle(stream)),
+              'wt_particle_gather')
 
     def pack_particles_slots(self, pos, spd, ids, q16, q8, stream=None):
-        check(self.lib.wt_pack_particles_slots(self.gp, _ptr(pos), _ptr(spd), _ptr(ids), _ptr(q16), _ptr(q8),
-                                               int(pos.shape[0]), stream_handle(stream)), 'wt_pack_particles_slots')
+        n, (pp, sp, ip) = self._particles(pos, spd, ids=ids)
+        check(self.lib.wt_pack_particles_slots(self.gp, pp, sp, ip,
+                                               self._arg(q16, 'q16', (torch.int16, torch.uint16), min_numel=4 * n),
+                                               self._arg(q8, 'q8', (torch.uint8,), min_numel=n), n,
+                                               stream_handle(stream)), 'wt_pack_particles_slots')
 
     def particle_order(self, pos, order, workspace=None, stream=None):
         """order[:] = particle ids sorted by cell index; returns the workspace for reuse."""
+        if not isinstance(pos, torch.Tensor) or pos.dim() != 2:
+            raise ValueError('pos must be an (n, 4) float32 tensor')
         n = int(pos.shape[0])
+        pp, op = self._arg(pos, 'pos', F32, (n, 4)), self._arg(order, 'order', I32, (n,))
         need = check(self.lib.wt_particle_order_workspace_bytes(n), 'wt_particle_order_workspace_bytes')
         if workspace is None or workspace.numel() < need:
             workspace = torch.empty(need, dtype=torch.uint8, device=self.device)
-        check(self.lib.wt_particle_order(self.gp, _ptr(pos), n, _ptr(order), _ptr(workspace), workspace.numel(),
-                                         stream_handle(stream)), 'wt_particle_order')
+        wp = self._arg(workspace, 'workspace', (torch.uint8,))
+        check(self.lib.wt_particle_order(self.gp, pp, n, op, wp, workspace.numel(), stream_handle(stream)),
+              'wt_particle_order')
         return workspace
 
     def pack_particles(self, pos, spd, q16, q8, stream=None):
-        check(self.lib.wt_pack_particles(self.gp, _ptr(pos), _ptr(spd), _ptr(q16), _ptr(q8),
-                                         int(pos.shape[0]), stream_handle(stream)), 'wt_pack_particles')
+        n, (pp, sp) = self._particles(pos, spd)
+        check(self.lib.wt_pack_particles(self.gp, pp, sp,
+                                         self._arg(q16, 'q16', (torch.int16, torch.uint16), min_numel=4 * n),
+                                         self._arg(q8, 'q8', (torch.uint8,), min_numel=n), n,
+                                         stream_handle(stream)), 'wt_pack_particles')
+
+    def slice_dims(self, axis):
+        """(w, h) of a slice perpendicular to axis (lbm.py:455-456)."""
+        if int(axis) not in SLICE_AXES:
+            raise ValueError(f'slice axis must be 0, 1 or 2, got {axis}')
+        a, b = SLICE_AXES[int(axis)]
+        return getattr(self, a), getattr(self, b)
 
     def slice(self, out, axis, index, field, uin, lref, vmin, vmax, stream=None):
-        check(self.lib.wt_slice(self.gp, _ptr(out), int(axis), int(index), int(field), float(uin),
-                                float(lref), float(vmin), float(vmax), stream_handle(stream)), 'wt_slice')
+        """out[0:2wh] = field bytes and fluid bytes of the plane (slice_kernel, lbm.py:278-318)."""
+        w, h = self.slice_dims(axis)
+        op = self._arg(out, 'slice output', (torch.uint8,), min_numel=2 * w * h)
+        check(self.lib.wt_slice(self.gp, op, int(axis), int(index), int(field), float(uin), float(lref),
+                                float(vmin), float(vmax), stream_handle(stream)), 'wt_slice')
 
     def cp(self, verts, norms, out, uin, cp_range, stream=None):
-        check(self.lib.wt_cp(self.gp, _ptr(verts), _ptr(norms), _ptr(out), int(verts.shape[0]), float(uin),
+        if not isinstance(verts, torch.Tensor) or verts.dim() != 2:
+            raise ValueError('verts must be an (nv, 3) float32 tensor')
+        nv = int(verts.shape[0])
+        check(self.lib.wt_cp(self.gp, self._arg(verts, 'verts', F32, (nv, 3)),
+                             self._arg(norms, 'norms', F32, (nv, 3)),
+                             self._arg(out, 'cp output', (torch.int8,), min_numel=nv), nv, float(uin),
                              float(cp_range), stream_handle(stream)), 'wt_cp')
 
 

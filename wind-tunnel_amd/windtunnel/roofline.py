@@ -5,7 +5,9 @@ sweep of its launch shapes.
     python -m windtunnel.roofline [--forms fp32,fp16] [--steps 100] [--sweep] [--copy-gb 4]
                                   [--particles N --fused fp32]
 
-Prints one JSON object per measurement. Algorithmic bytes per cell of the step:
+Each form runs the launch shape tuned for its size class (windtunnel/tuning.json,
+as sessions and bench.py do), or with --sweep a list of shapes. Prints one JSON
+object per measurement. Algorithmic bytes per cell of the step:
 19 populations read + 19 written, 1 class byte, 16 B macroscopic store
 (FP32 169, FP16/BF16 93, FP64 321). With --particles, the forms named by
 --fused are timed as wt_step_advect launches (the step fused with the slot
@@ -20,7 +22,7 @@ import sys
 import numpy as np
 import torch
 
-from . import _lib, geometry
+from . import _lib, autotune, geometry
 from .tunnel import Tunnel, get_launch_shape, set_launch_shape
 
 BYTES_PER_CELL = {'fp32': 169, 'fp16': 93, 'bf16': 93, 'fp64': 321}
@@ -139,7 +141,10 @@ def main(argv=None):
         print(json.dumps(copy_bandwidth(args.copy_gb)), flush=True)
     mask = c2_mask(dims)
     for form in args.forms.split(','):
-        shapes = [(1, 0), (1, 1), (1, 2), (1, 3), (2, 0), (2, 2), (4, 0)] if args.sweep else [get_launch_shape(form)]
+        # without --sweep: the shape tuned for this size class, the one sessions and bench.py run
+        n = dims[0] * dims[1] * dims[2]
+        shapes = ([(1, 0), (1, 1), (1, 2), (1, 3), (2, 0), (2, 2), (4, 0)] if args.sweep
+                  else [autotune.shape_for(form, n)])
         t = None
         old = get_launch_shape(form)
         for cpt, remap in shapes:

@@ -90,6 +90,7 @@ def run_gpu(form, steps, warmup, mask, counts, lref):
     plain = [ev[k][0].elapsed_time(ev[k][1]) for k in sampled]
     step_ms = sum(plain) / max(1, len(plain))
     cd, cl = s.coefficients()
+    viewer = s.viewer_coefficients()
     n = s.n
     nbytes = BYTES_PER_CELL[form] * n + (BYTES_PER_PARTICLE * s.num_p if s.fuse else 0)
     res = {
@@ -100,7 +101,7 @@ def run_gpu(form, steps, warmup, mask, counts, lref):
         'kernel': f"k_step_adv<{FORM_IDS[form]}, 1> (step + previous step's advection)" if s.fuse
                   else f'k_step<{FORM_IDS[form]}, 1> (fused stream/collide)',
         'achieved_gbs': nbytes / (step_ms * 1e-3) / 1e9,
-        'cd': float(cd), 'cl': float(cl),
+        'cd': float(cd), 'cl': float(cl), 'viewer_cd_cl': None if viewer is None else [float(v) for v in viewer],
     }
     del s
     torch.cuda.empty_cache()
@@ -240,6 +241,10 @@ def main():
     # differ; storage precision is compared on time averages of the sphere case
     # (windtunnel/precision.py)
     line['cd_cl_last_frame'] = {f: [round(v['cd'], 4), round(v['cl'], 4)] for f, v in results.items()}
+    # the viewer's average of every frame (800-step settle, then a running mean
+    # growing into a ~2500-step exponential window, web/js/main.js:336-351)
+    line['cd_cl_viewer_average'] = {f: None if v['viewer_cd_cl'] is None else [round(x, 4) for x in v['viewer_cd_cl']]
+                                    for f, v in results.items()}
     if c4:
         line['config4_batch'] = {'value': round(c4['mlups'], 1), 'unit': 'MLUPS', 'tunnels': c4['tunnels'],
                                  'cells': c4['cells'], 'steps': c4['steps'], 'ms_per_step': round(c4['ms_per_step'], 4),

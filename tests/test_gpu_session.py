@@ -81,3 +81,29 @@ def test_roofline_times_fused_launches(gpu):
     t, r = roofline.step_timing('fp32', dims, np.zeros(dims[::-1], np.uint8), steps=3, warmup=2, particles=1000)
     assert r['what'] == 'step_advect' and r['bytes_per_launch'] == 169 * t.n + 40 * 1000
     assert r['ms_median'] > 0 and t.mac.data_ptr() == t.grid.mac
+
+
+def test_viewer_average_equals_frame_by_frame_reads(gpu):
+    """Session.viewer_coefficients (forces logged on the device each frame) gives
+    exactly the viewer's average of the forces read synchronously frame by frame."""
+    from windtunnel.coefficients import ForceAverage
+    from windtunnel.session import FRAME_LOG, Session
+    from windtunnel import geometry
+    dims = (48, 20, 20)
+    mask = geometry.sphere(dims, (14, 10, 10), 6)[0]
+    s = Session(dims, mask, 'fp32', 0.05, 0.52, 0.16, frame_every=5, frontal=113)
+    ref = ForceAverage(0.05, 113)
+    for _ in range(1000):
+        before = s.frames
+        s.step()
+        if s.frames != before:
+            torch.cuda.synchronize()
+            f = s.forces_host.numpy()
+            ref.add(f[0], f[1], 5)
+    assert s.frames == 200 and ref.settled
+    assert s.viewer_coefficients() == (ref.cd, ref.cl)
+    t = Session(dims, mask, 'fp32', 0.05, 0.52, 0.16, frame_every=1, frontal=113)
+    for _ in range(FRAME_LOG + 1):
+        t.step()
+    with pytest.raises(RuntimeError, match='log holds'):
+        t.viewer_coefficients()

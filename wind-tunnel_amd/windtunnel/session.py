@@ -16,10 +16,13 @@ import numpy as np
 import torch
 
 from ._lib import WT_FLAG_FORCES
+from .coefficients import ForceAverage
 from .tunnel import Tunnel
 
 # storage forms whose sessions fuse the particle advection into the next step
 FUSED_FORMS = ('fp32',)
+# frames whose forces the device-side log holds between two reads of the average
+FRAME_LOG = 4096
 
 # slice field ranges of the viewer (web/js/main.js:15-20)
 FIELD_RANGES = {0: (0.0, 1.6), 1: (-1.2, 1.2), 2: (0.0, 12.0), 3: (-0.4, 1.4)}
@@ -88,7 +91,12 @@ class Session:
             self.slice_dev.append(torch.zeros(2 * w * h, dtype=torch.uint8, device=device))
             self.slice_host.append(torch.zeros(2 * w * h, dtype=torch.uint8).pin_memory())
         self.forces_host = torch.zeros(4, dtype=torch.float64).pin_memory()
-        self.force_log = []
+        # the force of every frame, logged on the device (one 32-byte copy per frame)
+        # so that the viewer's Cd/Cl average costs no synchronization while running
+        self.frame_forces = torch.zeros((FRAME_LOG, 4), dtype=torch.float64, device=device)
+        self.frames = 0
+        self._averaged = 0
+        self.average = ForceAverage(self.uin, self.frontal)
 
     @property
     def n(self):
@@ -173,11 +181,31 @@ class Session:
         return pos, spd
 
     def emit_frame(self, stream=None):
-        for (axis, index, field), dev, host in zip(self.slices, self.slice_dev, self.slice_host):
-            vmin, vmax = FIELD_RANGES.get(field, (0.0, 1.0))
-            self.T.slice(dev, axis, index, field, self.uin, self.lref, vmin, vmax, stream=stream)
-            host.copy_(dev, non_blocking=True)
-        self.forces_host.copy_(self.T.sums, non_blocking=True)
+        s = stream if stream is not None else torch.cuda.current_stream(self.T.device)
+        with torch.cuda.stream(s):
+            for (axis, index, field), dev, host in zip(self.slices, self.slice_dev, self.slice_host):
+                vmin, vmax = FIELD_RANGES.get(field, (0.0, 1.0))
+                self.T.slice(dev, axis, index, field, self.uin, self.lref, vmin, vmax, stream=s)
+                host.copy_(dev, non_blocking=True)
+            self.forces_host.copy_(self.T.sums, non_blocking=True)
+            self.frame_forces[self.frames % FRAME_LOG].copy_(self.T.sums, non_blocking=True)
+        self.frames += 1
+
+    def viewer_coefficients(self):
+        """(Cd, Cl) as the reference viewer shows them: every frame's force folded
+        into windtunnel.coefficients.ForceAverage (web/js/main.js:336-351), frames
+        of frame_every steps. None while settling or after divergence. Synchronizes."""
+        if self.frames - self._averaged > FRAME_LOG:
+            raise RuntimeError(f'{self.frames - self._averaged} frames since the last read; the log holds '
+                               f'{FRAME_LOG}')
+        torch.cuda.synchronize(self.T.device)
+        log = self.frame_forces.cpu().numpy()
+        for k in range(self._averaged, self.frames):
+            f = log[k % FRAME_LOG]
+            self.average.add(f[0], f[1], self.frame_every)
+        self._averaged = self.frames
+        a = self.average
+        return (a.cd, a.cl) if a.settled and not a.diverged else None
 
     def coefficients(self):
         """Cd, Cl from the last copied forces (web/js/main.js:336-339); call after a sync."""

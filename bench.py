@@ -171,8 +171,9 @@ def main():
     ap.add_argument('--serve-seconds', type=float, default=30.0)
     args = ap.parse_args()
     if args.gpus != 1:
-        print(f'bench.py measures one GPU: this build has no multi-GPU path (asked for --gpus {args.gpus}). '
-              'Independent tunnels shard one per GPU with no communication; see DESIGN.md.', file=sys.stderr)
+        print(f'bench.py measures one GPU: there is no multi-GPU path (asked for --gpus {args.gpus}). '
+              'Independent tunnels shard one per GPU with no communication; one tunnel larger than a GPU '
+              'would shard as x-slabs exchanging a one-cell halo of 5 populations per face.', file=sys.stderr)
         sys.exit(2)
     import torch
     if not torch.cuda.is_available():

@@ -90,3 +90,35 @@ def test_batch_rejects_mixed_forms(gpu):
     from windtunnel.tunnel import Tunnel
     with pytest.raises(ValueError):
         Batch([Tunnel((8, 8, 8), 'fp32'), Tunnel((8, 8, 8), 'fp16')], [(0.05, 0.52, 0.16)] * 2)
+
+
+def test_batch_with_a_large_tunnel_equals_alone(gpu):
+    """A 2.4M-cell tunnel (10 first-level reduction chunks) batched with three
+    small ones (1 chunk each): every tunnel's force/mass sums equal the same
+    tunnel stepped alone, bit for bit."""
+    from windtunnel import geometry
+    from windtunnel._lib import WT_FLAG_FORCES, WT_FLAG_MASS
+    from windtunnel.batch import Batch
+    from windtunnel.tunnel import Tunnel
+    big_dims, big_p = (192, 112, 112), (0.05, 0.52, 0.16)
+    assert big_dims[0] * big_dims[1] * big_dims[2] > 8 * 1024 * 256
+    big_mask = geometry.sphere(big_dims, (60, 56, 56), 20)[0]
+
+    def big():
+        t = Tunnel(big_dims, 'fp32')
+        t.init(big_p[0])
+        t.set_solid(big_mask)
+        return t
+    alone = _make('fp32')[:3] + [big()]
+    mixed = _make('fp32')[:3] + [big()]
+    params = PARAMS[:3] + [big_p]
+    b = Batch(mixed, params)
+    flags = WT_FLAG_FORCES | WT_FLAG_MASS
+    for k in range(4):
+        b.step(flags)
+        for t, p in zip(alone, params):
+            t.step(1, *p, flags=flags)
+        torch.cuda.synchronize()
+        bs = b.last_sums().cpu().numpy()
+        for j, t in enumerate(alone):
+            assert _same([t.sums.cpu().numpy()], [bs[j]]), (k, t.nx)

@@ -5,6 +5,10 @@ __graft_entry__.py and by the cpu_baseline leg of bench.py, never by the
 product package. See oracle/wt_oracle.cpp for what each function restates
 (server/lbm.py line references) and for the deliberate choices where the
 reference is nondeterministic.
+
+Parity unpinned: the respawn random stream of advect (replaced; checked by
+distribution) and Warp's last-ulp FP32 rounding, which cannot be observed
+without the reference's runtime (warp-lang 1.17.0, not installed).
 """
 import ctypes
 import os

@@ -32,6 +32,16 @@
 // Reduced-precision storage forms (FP16 / BF16 deviations g_i = f_i - w_i,
 // collision in FP32) are restated with software round-to-nearest-even
 // conversions so they can be compared bit for bit.
+//
+// Parity unpinned (no reference output exists to compare against, because
+// warp-lang 1.17.0 is not installed and the reference cannot run here):
+//   * the respawn stream of advect (lbm.py:256-259): replaced by this file's
+//     hash; its respawn positions are checked by distribution only;
+//   * Warp's last-ulp FP32 behaviour in every function above (its FMA
+//     contraction, division and sqrt rounding, wp.lerp as a*(1-t) + b*t):
+//     this file rounds each IEEE operation once, in the reference's order.
+// Every other function is pinned by closed forms, invariants, brute force on
+// tiny inputs or golden vectors (tests/test_oracle.py, tests/test_lib_cpu.py).
 #include <omp.h>
 
 #include <algorithm>

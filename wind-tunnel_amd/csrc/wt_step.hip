@@ -24,6 +24,11 @@
 // faces (deterministic Jacobi rule; the reference reads mac[] in the same
 // launch that writes it, lbm.py:87 vs lbm.py:166).
 //
+// k_step_adv launches a step together with the particle advection of the
+// previous step (advect_kernel, lbm.py:233-263, shared code in wt_particles.h):
+// its first workgroups advect from the macroscopic buffer that step wrote, the
+// rest run the step's tiles into the other buffer.
+//
 // The whole library is compiled with -ffp-contract=off: each operation is
 // rounded once, in the order the reference's generated source uses, so the
 // FP32 path can be compared bit for bit with the CPU restatement.

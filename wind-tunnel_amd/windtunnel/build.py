@@ -19,7 +19,10 @@ INCLUDE = os.path.join(REPO, 'include')
 LIB = os.path.join(PKG, 'libwt.so')
 ARCH = os.environ.get('WT_OFFLOAD_ARCH', 'gfx950')
 
-HIP_SOURCES = ['wt_step.hip', 'wt_aux.hip']
+HIP_SOURCES = ['wt_step.hip', 'wt_step_fp32.hip', 'wt_aux.hip']
+# per-source compiler options: the FP32 step kernels schedule for instruction-level
+# parallelism (measured 1.7 % faster at 33.5M cells; the FP16 kernels 8 % slower)
+SOURCE_FLAGS = {'wt_step_fp32.hip': ['-mllvm', '-amdgpu-sched-strategy=max-ilp']}
 CPP_SOURCES = ['wt_voxel.cpp']
 
 COMMON = ['-O3', '-std=c++17', '-fPIC', '-ffp-contract=off', '-Wall', '-Wno-unused-function',
@@ -54,7 +57,7 @@ def build(verbose=False, jobs=None):
         objs, cmds = [], []
         for src in HIP_SOURCES:
             o = os.path.join(tmp, src + '.o')
-            cmds.append([cc, *COMMON, *HIP_FLAGS, '-c', os.path.join(CSRC, src), '-o', o])
+            cmds.append([cc, *COMMON, *HIP_FLAGS, *SOURCE_FLAGS.get(src, []), '-c', os.path.join(CSRC, src), '-o', o])
             objs.append(o)
         for src in CPP_SOURCES:
             o = os.path.join(tmp, src + '.o')
@@ -76,7 +79,8 @@ def build(verbose=False, jobs=None):
                 print(out, flush=True)
         if errors:
             raise RuntimeError('libwt build failed:\n' + '\n\n'.join(errors))
-        _run([cc, '-shared', '-fPIC', '--offload-arch=' + ARCH, *objs, '-o', LIB + '.tmp'], verbose)
+        # -z defs: an instantiation missing from every unit fails here, not at load time
+        _run([cc, '-shared', '-fPIC', '--offload-arch=' + ARCH, '-Wl,-z,defs', *objs, '-o', LIB + '.tmp'], verbose)
         os.replace(LIB + '.tmp', LIB)
     finally:
         shutil.rmtree(tmp, ignore_errors=True)

@@ -950,7 +950,6 @@ extern template void launch_step_adv_f<WT_FP32>(const StepDesc& d, const AdvectA
 extern template void launch_batch_f<WT_FP32>(const StepDesc* dd, int nd, uint32_t ntile, int parity, int cpt,
                                              int remap, int flags, hipStream_t s);
 
-
 static int64_t nslots_of(int64_t n) { return (n + SLOT - 1) / SLOT; }
 static int64_t nchunks_of(int64_t n) { return (nslots_of(n) + CHUNK - 1) / CHUNK; }
 

@@ -3,7 +3,6 @@ admission, and every session's per-step force stream bit-identical alone and
 in the mix."""
 import numpy as np
 import pytest
-import torch
 
 pytestmark = pytest.mark.gpu
 

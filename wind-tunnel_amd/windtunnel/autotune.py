@@ -56,7 +56,6 @@ def shape_for(form_name, cells):
 
 
 def tune(forms, steps=40):
-    import numpy as np
     import torch
     from . import geometry
     from .roofline import step_timing

@@ -14,8 +14,6 @@ import argparse
 import json
 import sys
 
-import torch
-
 from .serving import Server, mixed_workload
 
 

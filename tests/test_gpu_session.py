@@ -52,7 +52,7 @@ def test_resorted_session_equals_sorted_once(gpu):
     assert np.isfinite(a['pos']).all() and (a['spd'] > 0).all()
 
 
-@pytest.mark.parametrize('form', ['fp32', 'fp16'])
+@pytest.mark.parametrize('form', ['fp32', 'fp16', 'bf16', 'fp64'])
 def test_fused_advection_equals_separate_launches(gpu, form):
     """wt_step_advect (step k + 1 fused with the advection of step k, two
     alternating mac buffers) gives what wt_step followed by wt_advect_slots

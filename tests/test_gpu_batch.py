@@ -44,7 +44,7 @@ def _same(a, b):
                for x, y in zip(a, b))
 
 
-@pytest.mark.parametrize('form', ['fp32', 'fp16'])
+@pytest.mark.parametrize('form', ['fp32', 'fp16', 'bf16', 'fp64'])
 def test_batch_equals_alone(gpu, form):
     from windtunnel._lib import WT_FLAG_FORCES, WT_FLAG_MASS
     from windtunnel.batch import Batch

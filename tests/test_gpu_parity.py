@@ -186,16 +186,25 @@ def test_fused_force_equals_standalone(gpu):
 # addresses (the form tunnels beyond 32-bit byte offsets take)
 @pytest.mark.parametrize('cpt,remap', [(1, 0), (2, 1), (4, 1), (4, 0), (1, 2), (2, 3), (1, 4), (2, 5)])
 def test_launch_shape_does_not_change_results(gpu, cpt, remap):
+    _shape_invariance('fp32', cpt, remap)
+
+
+@pytest.mark.parametrize('cpt,remap', [(1, 0), (2, 3), (4, 1), (1, 4)])
+def test_launch_shape_does_not_change_results_fp16(gpu, cpt, remap):
+    _shape_invariance('fp16', cpt, remap)
+
+
+def _shape_invariance(form, cpt, remap):
     from windtunnel import tunnel as T
     from windtunnel._lib import WT_FLAG_FORCES, WT_FLAG_MASS
     dims = (70, 26, 22)    # n not a multiple of 256 or 1024
     mask = _car_mask(dims, seed=6)
     res = []
     for shape in ((1, 1), (cpt, remap)):
-        t = _tunnel(dims, 'fp32', mask, 0.05)
+        t = _tunnel(dims, form, mask, 0.05)
         t.shape = shape
         t.step(11, 0.05, 0.52, 0.17, flags=WT_FLAG_FORCES | WT_FLAG_MASS)
-        assert T.get_launch_shape('fp32') == shape
+        assert T.get_launch_shape(form) == shape
         res.append((t.raw_populations().cpu().numpy(), t.mac.cpu().numpy(), t.sums.cpu().numpy()))
     for a, b in zip(res[0], res[1]):
         assert _raw_equal(a, b)

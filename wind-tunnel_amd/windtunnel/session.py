@@ -41,7 +41,7 @@ def inlet_rake(dims, ground=True):
 class Session:
     def __init__(self, dims, mask, form='fp32', uin=0.05, tau=0.51, smag=0.16, particles=0,
                  slices=(), frame_every=10, lref=40.0, seed=0, device='cuda', ground=True, frontal=1,
-                 fuse=None):
+                 fuse=None, side_copies=True):
         self.dims = tuple(int(d) for d in dims)
         self.uin, self.tau, self.smag = float(uin), float(tau), float(smag)
         self.frame_every = int(frame_every)
@@ -94,6 +94,12 @@ class Session:
         # the force of every frame, logged on the device (one 32-byte copy per frame)
         # so that the viewer's Cd/Cl average costs no synchronization while running
         self.frame_forces = torch.zeros((FRAME_LOG, 4), dtype=torch.float64, device=device)
+        # device-to-host copies of slices and forces go out on a side stream (default
+        # priority), so the copy engine overlaps them with the next steps; the next
+        # frame's slices and the next force step wait for them (an in-stream event,
+        # long satisfied by then). side_copies=False keeps them on the step's stream.
+        self.copy_stream = torch.cuda.Stream(device=self.T.device) if side_copies else None
+        self._copied = None
         self.frames = 0
         self._averaged = 0
         self.average = ForceAverage(self.uin, self.frontal)
@@ -113,6 +119,8 @@ class Session:
         want_force = frame if force is None else force
         main = stream if stream is not None else torch.cuda.current_stream(self.T.device)
         flags = WT_FLAG_FORCES if want_force else 0
+        if want_force and self._copied is not None:
+            main.wait_event(self._copied)        # the reduction rewrites the sums being copied
         if self.fuse:
             b = self.step_count & 1
             self.T.set_mac(self.macs[b])
@@ -144,7 +152,7 @@ class Session:
         if frame:
             self.emit_frame(main)
         elif want_force:
-            self.forces_host.copy_(self.T.sums, non_blocking=True)
+            self._copy_out(main, slices=False)
         self.step_count += 1
 
     def _sort_if_due(self, stream):
@@ -183,13 +191,31 @@ class Session:
     def emit_frame(self, stream=None):
         s = stream if stream is not None else torch.cuda.current_stream(self.T.device)
         with torch.cuda.stream(s):
-            for (axis, index, field), dev, host in zip(self.slices, self.slice_dev, self.slice_host):
+            if self._copied is not None:
+                s.wait_event(self._copied)      # the last frame's copies still read slice_dev
+            for (axis, index, field), dev in zip(self.slices, self.slice_dev):
                 vmin, vmax = FIELD_RANGES.get(field, (0.0, 1.0))
                 self.T.slice(dev, axis, index, field, self.uin, self.lref, vmin, vmax, stream=s)
-                host.copy_(dev, non_blocking=True)
-            self.forces_host.copy_(self.T.sums, non_blocking=True)
             self.frame_forces[self.frames % FRAME_LOG].copy_(self.T.sums, non_blocking=True)
         self.frames += 1
+        self._copy_out(s, slices=True)
+
+    def _copy_out(self, s, slices):
+        """Queue the copies of the slices (if asked) and the force sums into pinned
+        host memory after the work queued so far on stream s."""
+        cs = self.copy_stream or s
+        if cs is not s:
+            ready = torch.cuda.Event()
+            ready.record(s)
+            cs.wait_event(ready)
+        with torch.cuda.stream(cs):
+            if slices:
+                for dev, host in zip(self.slice_dev, self.slice_host):
+                    host.copy_(dev, non_blocking=True)
+            self.forces_host.copy_(self.T.sums, non_blocking=True)
+            if cs is not s:
+                self._copied = torch.cuda.Event()
+                self._copied.record(cs)
 
     def viewer_coefficients(self):
         """(Cd, Cl) as the reference viewer shows them: every frame's force folded

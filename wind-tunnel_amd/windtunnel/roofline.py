@@ -74,7 +74,7 @@ class _Rake:
         pos = torch.from_numpy(d).to(t.device)
         spd = torch.ones(num_p, dtype=torch.float32, device=t.device)
         ids = torch.arange(num_p, dtype=torch.int32, device=t.device)
-        # slots in cell order, as a session keeps them (re-sorted every 50 steps)
+        # slots in cell order, as a session keeps them (re-sorted every 100 steps)
         order = torch.empty(num_p, dtype=torch.int32, device=t.device)
         t.particle_order(pos, order, None)
         self.pos, self.spd, self.ids = torch.empty_like(pos), torch.empty_like(spd), torch.empty_like(ids)

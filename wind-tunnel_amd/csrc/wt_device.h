@@ -75,12 +75,10 @@ template <> struct Form<WT_FP32> {
 __device__ inline float h2f(uint16_t b) { return (float)__builtin_bit_cast(_Float16, b); }
 __device__ inline uint16_t f2h(float f) { return __builtin_bit_cast(uint16_t, (_Float16)f); }
 __device__ inline float bf2f(uint16_t b) { return __uint_as_float((uint32_t)b << 16); }
-__device__ inline uint16_t f2bf(float f) {
-  uint32_t x = __float_as_uint(f);
-  if ((x & 0x7fffffffu) > 0x7f800000u) return (uint16_t)((x >> 16) | 0x40u);
-  x += 0x7fffu + ((x >> 16) & 1u);
-  return (uint16_t)(x >> 16);
-}
+// Round to nearest even, NaN kept quiet: gfx950's v_cvt_pk_bf16_f32, one
+// instruction where the bit arithmetic of the same rounding takes five or six
+// (x += 0x7fff + lsb, x >> 16; NaN: (x >> 16) | 0x40, the oracle's form).
+__device__ inline uint16_t f2bf(float f) { return __builtin_bit_cast(uint16_t, (__bf16)f); }
 
 template <> struct Form<WT_FP16> {
   typedef uint16_t elem;

@@ -132,6 +132,7 @@ def main(argv=None):
     ap.add_argument('--dims', default='512,256,256')
     ap.add_argument('--steps', type=int, default=100)
     ap.add_argument('--sweep', action='store_true')
+    ap.add_argument('--shapes', default='', help="launch shapes to time, e.g. '1,2 2,10' (cpt,remap pairs)")
     ap.add_argument('--copy-gb', type=float, default=4.0)
     ap.add_argument('--particles', type=int, default=0)
     ap.add_argument('--fused', default='fp32', help='forms timed as fused step + advection with --particles')
@@ -145,6 +146,8 @@ def main(argv=None):
         n = dims[0] * dims[1] * dims[2]
         shapes = ([(1, 0), (1, 1), (1, 2), (1, 3), (2, 0), (2, 2), (4, 0)] if args.sweep
                   else [autotune.shape_for(form, n)])
+        if args.shapes:
+            shapes = [tuple(int(v) for v in sh.split(',')) for sh in args.shapes.split()]
         t = None
         old = get_launch_shape(form)
         for cpt, remap in shapes:

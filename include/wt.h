@@ -208,6 +208,12 @@ int wt_cp(const wt_grid* g, const float* verts, const float* norms, int8_t* out,
  * of population i; bit 27+i moving-belt term; bit 45+i link i counts toward the
  * force (i = 1..18). Mirrors lbm.py:81-84, 114-124, 185-188. */
 int wt_link_classes(const wt_grid* g, uint64_t* out_dev, void* stream);
+/* Self-test of the BF16 storage rounding: the device conversion the step
+ * kernels store with against round-to-nearest-even in bit arithmetic, over all
+ * 2^32 float bit patterns. counts_dev[0] (uint64) = differing non-NaN inputs,
+ * counts_dev[1] = differing NaN inputs; both 0 when the two agree. No counterpart
+ * in the reference, whose storage is FP32 only. */
+int wt_selftest_bf16(uint64_t* counts_dev, void* stream);
 
 /* ---- measurement ---------------------------------------------------------- */
 /* Streaming device-to-device copy of `bytes` (multiple of 16, 16-byte aligned

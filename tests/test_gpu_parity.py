@@ -225,6 +225,19 @@ def test_global_addressing_matches_buffer_addressing(gpu, form):
         assert _raw_equal(a, b)
 
 
+def test_bf16_store_rounding_equals_the_oracles_for_every_float(gpu):
+    """The BF16 form stores with the device's float -> bf16 conversion; the oracle
+    rounds in bit arithmetic (nearest even, NaN quieted; pinned in test_oracle.py).
+    The library's self-test compares the two over all 2^32 float bit patterns."""
+    import ctypes
+    from windtunnel import _lib
+    counts = torch.full((2,), -1, dtype=torch.int64, device='cuda')
+    stream = ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+    _lib.check(_lib.load().wt_selftest_bf16(ctypes.c_void_p(counts.data_ptr()), stream), 'wt_selftest_bf16')
+    torch.cuda.synchronize()
+    assert counts.cpu().tolist() == [0, 0]
+
+
 def test_repeatable(gpu):
     from windtunnel._lib import WT_FLAG_FORCES
     dims = (48, 20, 20)

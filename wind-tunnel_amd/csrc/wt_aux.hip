@@ -178,6 +178,28 @@ __global__ __launch_bounds__(ABLOCK) void k_stream_copy(const u32x4* __restrict_
   if (i < n16) __builtin_nontemporal_store(__builtin_nontemporal_load(src + i), dst + i);
 }
 
+// Self-test of the BF16 store rounding: f2bf (the hardware conversion the step
+// kernels use) against round-to-nearest-even in bit arithmetic (x + 0x7fff +
+// lsb, NaN quieted as (x >> 16) | 0x40: the oracle's rounding), over every
+// float bit pattern. counts[0]: non-NaN inputs that differ, counts[1]: NaN inputs.
+__device__ __forceinline__ uint16_t f2bf_bits(uint32_t x) {
+  if ((x & 0x7fffffffu) > 0x7f800000u) return (uint16_t)((x >> 16) | 0x40u);
+  return (uint16_t)((x + 0x7fffu + ((x >> 16) & 1u)) >> 16);
+}
+__global__ __launch_bounds__(ABLOCK) void k_zero_counts(unsigned long long* counts) {
+  if (threadIdx.x < 2) counts[threadIdx.x] = 0ull;
+}
+__global__ __launch_bounds__(ABLOCK) void k_selftest_bf16(unsigned long long* counts) {
+  const uint64_t stride = (uint64_t)gridDim.x * ABLOCK;
+  unsigned long long bad[2] = {0ull, 0ull};
+  for (uint64_t v = (uint64_t)blockIdx.x * ABLOCK + threadIdx.x; v < (1ull << 32); v += stride) {
+    const uint32_t x = (uint32_t)v;
+    if (f2bf(__uint_as_float(x)) != f2bf_bits(x)) bad[(x & 0x7fffffffu) > 0x7f800000u ? 1 : 0]++;
+  }
+  if (bad[0]) atomicAdd(&counts[0], bad[0]);
+  if (bad[1]) atomicAdd(&counts[1], bad[1]);
+}
+
 int check_small_grid(const wt_grid* g) {
   if (!g || !g->mac || !g->cls) return fail(-1, "null grid, mac or class buffer");
   if (g->nx < 3 || g->ny < 3 || g->nz < 3) return fail(-1, "grid dimensions must be at least 3");
@@ -217,6 +239,15 @@ int wt_stream_copy(void* dst, const void* src, int64_t bytes, int blocks, void* 
   hipLaunchKernelGGL(k_stream_copy, dim3((unsigned)((n16 + ABLOCK - 1) / ABLOCK)), dim3(ABLOCK), 0,
                      (hipStream_t)stream, (const u32x4*)src, (u32x4*)dst, n16);
   return check_launch("wt_stream_copy");
+}
+
+int wt_selftest_bf16(uint64_t* counts_dev, void* stream) {
+  if (!counts_dev || ((uintptr_t)counts_dev & 7)) return fail(-1, "counts need a non-null 8-byte aligned device buffer");
+  hipStream_t s = (hipStream_t)stream;
+  unsigned long long* c = reinterpret_cast<unsigned long long*>(counts_dev);
+  hipLaunchKernelGGL(k_zero_counts, dim3(1), dim3(ABLOCK), 0, s, c);
+  hipLaunchKernelGGL(k_selftest_bf16, dim3(16384), dim3(ABLOCK), 0, s, c);
+  return check_launch("wt_selftest_bf16");
 }
 
 int wt_advect(const wt_grid* g, float* pos, float* spd, int64_t num_p, float dt, float uin, float max_age,

@@ -78,6 +78,7 @@ SIGNATURES = {
     'wt_slice': (c_i32, [P(Grid), c_vp, c_i32, c_i32, c_i32, c_f32, c_f32, c_f32, c_f32, c_vp]),
     'wt_cp': (c_i32, [P(Grid), c_vp, c_vp, c_vp, c_i64, c_f32, c_f32, c_vp]),
     'wt_link_classes': (c_i32, [P(Grid), c_vp, c_vp]),
+    'wt_selftest_bf16': (c_i32, [c_vp, c_vp]),
     'wt_stream_copy': (c_i32, [c_vp, c_vp, c_i64, c_i32, c_vp]),
     'wt_voxelize': (c_i32, [c_vp, c_i64, c_i32, c_i32, c_i32, c_i32, c_vp, P(c_i64)]),
     'wt_ellipsoids': (c_i32, [c_vp, c_i32, c_i32, c_i32, c_i32, c_i32, c_vp, P(c_i64)]),

@@ -16,10 +16,11 @@ is reported beside it. roofline: algorithmic bytes of the step kernel
 byte, the 16 B macroscopic store) over its mean launch time, from HIP events on
 the stream the kernel runs on, bracketing every 10th launch of the timed
 region (steps 5, 15, 25, ...: force-free; a pair of events on every step slows
-the loop by ~0.7 %). With FP32 storage each step's launch also
-advects the particles of the previous step (k_step_adv), priced at 40 more
-bytes per particle (id 4, position 16 read + 16 written, speed 4 written).
-Every timed step's particles are advected inside the timed region.
+the loop by ~0.7 %). Each step's particles are advected by their own
+launch (k_advect) right after the step, inside the timed region; a session
+built with fuse=True instead advects them inside the next step's launch
+(k_step_adv, priced at 40 more bytes per particle: id 4, position 16 read + 16
+written, speed 4 written), which measured 0.05-0.4 % slower per step.
 cpu_baseline: the CPU restatement of the same step (oracle/, C++ with OpenMP)
 timed on this host on a bounded sample.
 """

@@ -19,8 +19,13 @@ from ._lib import WT_FLAG_FORCES
 from .coefficients import ForceAverage
 from .tunnel import Tunnel
 
-# storage forms whose sessions fuse the particle advection into the next step
-FUSED_FORMS = ('fp32',)
+# storage forms whose sessions fuse the particle advection into the next step's
+# launch (Session(fuse=True) asks for it in any form). None by default: with the
+# FP32 step kernels at 88 VGPRs (5 waves per SIMD, which the fused launch's
+# advection blocks then run at too) the fused loop measured 0.05-0.4 % slower per
+# step than a separate advection launch at 33.5M cells and 2^20 particles, and
+# FP16's 0.1-0.6 % slower.
+FUSED_FORMS = ()
 # frames whose forces the device-side log holds between two reads of the average
 FRAME_LOG = 4096
 

@@ -7,10 +7,10 @@ advects the particles with dt = 1 (advect_kernel, lbm.py:233-263), and on frame
 steps extracts slice planes (slice_kernel, lbm.py:278-318) and queues
 asynchronous copies of slices and forces into pinned host buffers.
 
-With FP32 storage the advection of step k runs inside the launch of step k + 1
-(wt_step_advect), saving ~1 % per step at 33.5M cells and 2^20 particles; with
-FP16 the same fusion measured 0.3-1.7 % slower, so it launches separately.
-Results are identical either way.
+With fuse=True the advection of step k runs inside the launch of step k + 1
+(wt_step_advect) instead of its own launch after step k. Results are identical
+either way; by default every form launches the advection separately, which
+measured faster at 33.5M cells and 2^20 particles (FUSED_FORMS below).
 """
 import numpy as np
 import torch

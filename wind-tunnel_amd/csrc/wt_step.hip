@@ -61,16 +61,19 @@ template <typename T> __device__ __forceinline__ T maxv(T a, T b) { return (a < 
 // the row-blocked layout: non-temporal stores are 7-8 % faster than cached ones
 // (FP32 and FP16), non-temporal loads 9 % slower (the x +- 1 and y/z pulls of
 // neighbouring waves reuse lines). With plain SoA planes the stores had measured
-// the other way round.
+// the other way round. The macroscopic field of a single-tunnel step goes out
+// non-temporal too: whole session steps (step, particle advection and slices,
+// which read it) measured 0.8 % faster for FP16 and the same for FP32 as with
+// cached stores. Batched steps keep it cached (FLAG_MAC_CACHED): non-temporal
+// made the 48-tunnel batch 0.8 % slower.
 #ifndef WT_LOAD_NT
 #define WT_LOAD_NT 0
 #endif
 #ifndef WT_STORE_NT
 #define WT_STORE_NT 1
 #endif
-#ifndef WT_MAC_NT
-#define WT_MAC_NT 0
-#endif
+// kernel flag bit beside WT_FLAG_FORCES / WT_FLAG_MASS, set only by the batch launch
+constexpr int FLAG_MAC_CACHED = 1 << 8;
 template <typename E> __device__ __forceinline__ E ld_stream(const E* p) {
   if constexpr (WT_LOAD_NT) return __builtin_nontemporal_load(p);
   else return *p;
@@ -81,9 +84,9 @@ template <typename E> __device__ __forceinline__ void st_stream(E* p, E v) {
 }
 
 typedef float f32x4 __attribute__((ext_vector_type(4)));
-__device__ __forceinline__ void st_mac(float4* p, float a, float b, float c, float d) {
-  if constexpr (WT_MAC_NT) __builtin_nontemporal_store(f32x4{a, b, c, d}, reinterpret_cast<f32x4*>(p));
-  else *p = make_float4(a, b, c, d);
+__device__ __forceinline__ void st_mac(float4* p, float a, float b, float c, float d, int flags) {
+  if (flags & FLAG_MAC_CACHED) *p = make_float4(a, b, c, d);
+  else __builtin_nontemporal_store(f32x4{a, b, c, d}, reinterpret_cast<f32x4*>(p));
 }
 
 // Signed sum c_i . u in index order, as cu_expr() emits it (lbm.py:48-53).
@@ -362,7 +365,7 @@ __device__ __forceinline__ void step_cell_general(const StepDesc& d, const typen
   const R zero = (R)0, one = (R)1;
 
   if ((c & CLS_CODE) != 0) {  // solid cell, lbm.py:81-83
-    st_mac(d.mac + idx, 0.f, 0.f, 0.f, 1.f);
+    st_mac(d.mac + idx, 0.f, 0.f, 0.f, 1.f, flags);
     shell_write<R>(shout, x, y, z, zero, zero, zero, one);
     return;
   }
@@ -396,7 +399,7 @@ __device__ __forceinline__ void step_cell_general(const StepDesc& d, const typen
 #pragma unroll
     for (int i = 0; i < Q; i++) v[i] = S::store(face_feq<R>(i, br, bux, buy, buz, bsq), i);
     store_all(v);
-    st_mac(d.mac + idx, (float)bux, (float)buy, (float)buz, (float)br);
+    st_mac(d.mac + idx, (float)bux, (float)buy, (float)buz, (float)br, flags);
     if (flags & WT_FLAG_MASS) acc.m += (double)br;
     return;
   }
@@ -451,7 +454,7 @@ __device__ __forceinline__ void step_cell_general(const StepDesc& d, const typen
 #pragma unroll
   for (int i = 0; i < Q; i++) st[i] = S::store(out[i], i);
   store_all(st);
-  st_mac(d.mac + idx, (float)ux, (float)uy, (float)uz, (float)rho);
+  st_mac(d.mac + idx, (float)ux, (float)uy, (float)uz, (float)rho, flags);
   shell_write<R>(shout, x, y, z, ux, uy, uz, rho);
   if (flags & WT_FLAG_MASS) acc.m += (double)rho;
   if ((flags & WT_FLAG_FORCES) && (c & CLS_FORCE)) add_force<FORM>(d.cls, idx, nx, nxy, st, acc);
@@ -525,7 +528,7 @@ __device__ __forceinline__ void step_cell_fast(const StepDesc& d, const typename
   } else {
     global_stores();
   }
-  st_mac(d.mac + idx, (float)ux, (float)uy, (float)uz, (float)rho);
+  st_mac(d.mac + idx, (float)ux, (float)uy, (float)uz, (float)rho, flags);
   if (!xface) shell_write<R>(shout, x, y, z, ux, uy, uz, rho);
   if (flags & WT_FLAG_MASS) acc.m += (double)rho;
 }
@@ -1251,13 +1254,14 @@ int wt_batch_step(void* workspace, const wt_batch_info* info, int parity, int fl
   const StepDesc* dd = (const StepDesc*)workspace;
   const RedDesc* rd = (const RedDesc*)((const char*)workspace + sizeof(StepDesc) * nt);
   const int f = flags & (WT_FLAG_FORCES | WT_FLAG_MASS);
+  const int fk = f | FLAG_MAC_CACHED;
   const uint32_t ntile = (uint32_t)info->total_tiles;
   const int remap = (info->remap & 1) | (((info->remap & 2) && (parity & 1)) ? 2 : 0);
   switch (form) {
-    case WT_FP32: launch_batch_f<WT_FP32>(dd, nt, ntile, parity, info->cpt, remap, f, s); break;
-    case WT_FP16: launch_batch_f<WT_FP16>(dd, nt, ntile, parity, info->cpt, remap, f, s); break;
-    case WT_BF16: launch_batch_f<WT_BF16>(dd, nt, ntile, parity, info->cpt, remap, f, s); break;
-    case WT_FP64: launch_batch_f<WT_FP64>(dd, nt, ntile, parity, info->cpt, remap, f, s); break;
+    case WT_FP32: launch_batch_f<WT_FP32>(dd, nt, ntile, parity, info->cpt, remap, fk, s); break;
+    case WT_FP16: launch_batch_f<WT_FP16>(dd, nt, ntile, parity, info->cpt, remap, fk, s); break;
+    case WT_BF16: launch_batch_f<WT_BF16>(dd, nt, ntile, parity, info->cpt, remap, fk, s); break;
+    case WT_FP64: launch_batch_f<WT_FP64>(dd, nt, ntile, parity, info->cpt, remap, fk, s); break;
   }
   if (f) {
     hipLaunchKernelGGL(k_reduce1, dim3((unsigned)info->max_chunks, (unsigned)nt), dim3(BLOCK), 0, s, rd);

@@ -59,21 +59,21 @@ template <typename T> __device__ __forceinline__ T maxv(T a, T b) { return (a < 
 
 // Cache policy of the population streams, measured on MI355X at 33.5M cells with
 // the row-blocked layout: non-temporal stores are 7-8 % faster than cached ones
-// (FP32 and FP16), non-temporal loads 9 % slower (the x +- 1 and y/z pulls of
-// neighbouring waves reuse lines). With plain SoA planes the stores had measured
-// the other way round. The macroscopic field of a single-tunnel step goes out
-// non-temporal too: whole session steps (step, particle advection and slices,
-// which read it) measured 0.8 % faster for FP16 and the same for FP32 as with
-// cached stores. Batched steps keep it cached (FLAG_MAC_CACHED): non-temporal
-// made the 48-tunnel batch 0.8 % slower.
+// (FP32 and FP16; with plain SoA planes they had measured the other way round).
+// Non-temporal loads of all planes are 9 % slower (the x +- 1 pulls of
+// neighbouring waves share lines); of the c_x = 0 planes alone, whose aligned
+// lines one wave reads whole, 0.9 % (FP32) and 0.5 % (FP16) faster per session
+// step. Storing the macroscopic field non-temporally made session steps (step,
+// particle advection and slices, which read the field) 0.8 % faster for FP16 and
+// left FP32 the same. Batched steps lose with both (0.35 and 0.8 %), so only
+// single-tunnel launches use them (run_tile's SOLO), on the raw-buffer paths; the
+// 64-bit global-address path loads every plane cached.
 #ifndef WT_LOAD_NT
 #define WT_LOAD_NT 0
 #endif
 #ifndef WT_STORE_NT
 #define WT_STORE_NT 1
 #endif
-// kernel flag bit beside WT_FLAG_FORCES / WT_FLAG_MASS, set only by the batch launch
-constexpr int FLAG_MAC_CACHED = 1 << 8;
 template <typename E> __device__ __forceinline__ E ld_stream(const E* p) {
   if constexpr (WT_LOAD_NT) return __builtin_nontemporal_load(p);
   else return *p;
@@ -84,9 +84,10 @@ template <typename E> __device__ __forceinline__ void st_stream(E* p, E v) {
 }
 
 typedef float f32x4 __attribute__((ext_vector_type(4)));
-__device__ __forceinline__ void st_mac(float4* p, float a, float b, float c, float d, int flags) {
-  if (flags & FLAG_MAC_CACHED) *p = make_float4(a, b, c, d);
-  else __builtin_nontemporal_store(f32x4{a, b, c, d}, reinterpret_cast<f32x4*>(p));
+template <bool NT>
+__device__ __forceinline__ void st_mac(float4* p, float a, float b, float c, float d) {
+  if constexpr (NT) __builtin_nontemporal_store(f32x4{a, b, c, d}, reinterpret_cast<f32x4*>(p));
+  else *p = make_float4(a, b, c, d);
 }
 
 // Signed sum c_i . u in index order, as cu_expr() emits it (lbm.py:48-53).
@@ -330,19 +331,22 @@ constexpr int LOAD_ORDER[Q] = {15, 5, 11, 14, 18, 3, 7, 10, 0, 1, 2, 4, 8, 9, 17
 #endif
 constexpr int CPOL_LD = WT_CPOL_LD;
 constexpr int CPOL_ST = WT_CPOL_ST;
+constexpr int CPOL_LD_NT = 2;   // loads of the c_x = 0 planes
 
 template <typename E> struct RawIO;
 template <> struct RawIO<float> {
-  __device__ static float ld(__amdgpu_buffer_rsrc_t r, uint32_t vo, uint32_t so) {
-    return __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(r, vo, so, CPOL_LD));
+  __device__ static float ld(__amdgpu_buffer_rsrc_t r, uint32_t vo, uint32_t so, bool nt) {
+    return __uint_as_float(nt ? __builtin_amdgcn_raw_buffer_load_b32(r, vo, so, CPOL_LD_NT)
+                              : __builtin_amdgcn_raw_buffer_load_b32(r, vo, so, CPOL_LD));
   }
   __device__ static void st(__amdgpu_buffer_rsrc_t r, uint32_t vo, uint32_t so, float v) {
     __builtin_amdgcn_raw_buffer_store_b32(__float_as_uint(v), r, vo, so, CPOL_ST);
   }
 };
 template <> struct RawIO<uint16_t> {
-  __device__ static uint16_t ld(__amdgpu_buffer_rsrc_t r, uint32_t vo, uint32_t so) {
-    return __builtin_amdgcn_raw_buffer_load_b16(r, vo, so, CPOL_LD);
+  __device__ static uint16_t ld(__amdgpu_buffer_rsrc_t r, uint32_t vo, uint32_t so, bool nt) {
+    return nt ? __builtin_amdgcn_raw_buffer_load_b16(r, vo, so, CPOL_LD_NT)
+              : __builtin_amdgcn_raw_buffer_load_b16(r, vo, so, CPOL_LD);
   }
   __device__ static void st(__amdgpu_buffer_rsrc_t r, uint32_t vo, uint32_t so, uint16_t v) {
     __builtin_amdgcn_raw_buffer_store_b16(v, r, vo, so, CPOL_ST);
@@ -350,7 +354,7 @@ template <> struct RawIO<uint16_t> {
 };
 
 // ---- one cell, general path (any mix of solid, face and near-wall lanes) ------------
-template <int FORM>
+template <int FORM, bool SOLO>
 __device__ __forceinline__ void step_cell_general(const StepDesc& d, const typename Form<FORM>::elem* fin,
                                                   typename Form<FORM>::elem* fout,
                                                   const Shell<typename Form<FORM>::real>& shin,
@@ -365,7 +369,7 @@ __device__ __forceinline__ void step_cell_general(const StepDesc& d, const typen
   const R zero = (R)0, one = (R)1;
 
   if ((c & CLS_CODE) != 0) {  // solid cell, lbm.py:81-83
-    st_mac(d.mac + idx, 0.f, 0.f, 0.f, 1.f, flags);
+    st_mac<SOLO>(d.mac + idx, 0.f, 0.f, 0.f, 1.f);
     shell_write<R>(shout, x, y, z, zero, zero, zero, one);
     return;
   }
@@ -399,7 +403,7 @@ __device__ __forceinline__ void step_cell_general(const StepDesc& d, const typen
 #pragma unroll
     for (int i = 0; i < Q; i++) v[i] = S::store(face_feq<R>(i, br, bux, buy, buz, bsq), i);
     store_all(v);
-    st_mac(d.mac + idx, (float)bux, (float)buy, (float)buz, (float)br, flags);
+    st_mac<SOLO>(d.mac + idx, (float)bux, (float)buy, (float)buz, (float)br);
     if (flags & WT_FLAG_MASS) acc.m += (double)br;
     return;
   }
@@ -434,7 +438,7 @@ __device__ __forceinline__ void step_cell_general(const StepDesc& d, const typen
         const uint32_t pull = pshift(i, nx, nxy) * ES + bias;
         const uint32_t wall = (uint32_t)ppos(opp(i)) * (uint32_t)nx * ES + bias;
         const uint32_t m = 0u - ((lk.bb >> i) & 1u);
-        f[i] = S::load(RawIO<E>::ld(rin, bo + (pull + (m & (wall - pull))), 0u), i);
+        f[i] = S::load(RawIO<E>::ld(rin, bo + (pull + (m & (wall - pull))), 0u, SOLO && cx(i) == 0), i);
       }
     } else {
       global_loads();
@@ -454,7 +458,7 @@ __device__ __forceinline__ void step_cell_general(const StepDesc& d, const typen
 #pragma unroll
   for (int i = 0; i < Q; i++) st[i] = S::store(out[i], i);
   store_all(st);
-  st_mac(d.mac + idx, (float)ux, (float)uy, (float)uz, (float)rho, flags);
+  st_mac<SOLO>(d.mac + idx, (float)ux, (float)uy, (float)uz, (float)rho);
   shell_write<R>(shout, x, y, z, ux, uy, uz, rho);
   if (flags & WT_FLAG_MASS) acc.m += (double)rho;
   if ((flags & WT_FLAG_FORCES) && (c & CLS_FORCE)) add_force<FORM>(d.cls, idx, nx, nxy, st, acc);
@@ -465,7 +469,7 @@ __device__ __forceinline__ void step_cell_general(const StepDesc& d, const typen
 // of population i is the lane's own element offset (one vector register shared
 // by all 38 loads and stores) plus the wave-uniform pshift(i) in a scalar
 // register (raw-buffer offsets, as above), with no per-lane address arithmetic.
-template <int FORM>
+template <int FORM, bool SOLO>
 __device__ __forceinline__ void step_cell_fast(const StepDesc& d, const typename Form<FORM>::elem* fin,
                                                typename Form<FORM>::elem* fout,
                                                const Shell<typename Form<FORM>::real>& shin,
@@ -502,7 +506,7 @@ __device__ __forceinline__ void step_cell_fast(const StepDesc& d, const typename
 #pragma unroll
         for (int k = 0; k < Q; k++) {
           const int i = LOAD_ORDER[k];
-          f[i] = S::load(RawIO<E>::ld(rin, bo, pshift(i, nx, nxy) * ES + bias), i);
+          f[i] = S::load(RawIO<E>::ld(rin, bo, pshift(i, nx, nxy) * ES + bias, SOLO && cx(i) == 0), i);
         }
       } else {
         global_loads();
@@ -528,12 +532,12 @@ __device__ __forceinline__ void step_cell_fast(const StepDesc& d, const typename
   } else {
     global_stores();
   }
-  st_mac(d.mac + idx, (float)ux, (float)uy, (float)uz, (float)rho, flags);
+  st_mac<SOLO>(d.mac + idx, (float)ux, (float)uy, (float)uz, (float)rho);
   if (!xface) shell_write<R>(shout, x, y, z, ux, uy, uz, rho);
   if (flags & WT_FLAG_MASS) acc.m += (double)rho;
 }
 
-template <int FORM>
+template <int FORM, bool SOLO>
 __device__ __forceinline__ void step_cell(const StepDesc& d, const typename Form<FORM>::elem* fin,
                                           typename Form<FORM>::elem* fout,
                                           const Shell<typename Form<FORM>::real>& shin,
@@ -552,9 +556,9 @@ __device__ __forceinline__ void step_cell(const StepDesc& d, const typename Form
   // fluid lanes away from walls, and fluid inlet/outlet lanes, take the fast path
   const bool simple = (c == 0) & (!face | xface) & (y != 0) & (y != ny - 1) & (z != 0) & (z != nz - 1);
   if (__all(simple))
-    step_cell_fast<FORM>(d, fin, fout, shin, shout, idx, pc, x, y, z, xface, flags, acc);
+    step_cell_fast<FORM, SOLO>(d, fin, fout, shin, shout, idx, pc, x, y, z, xface, flags, acc);
   else
-    step_cell_general<FORM>(d, fin, fout, shin, shout, idx, pc, c, x, y, z, face, flags, acc);
+    step_cell_general<FORM, SOLO>(d, fin, fout, shin, shout, idx, pc, c, x, y, z, face, flags, acc);
 }
 
 // ---- deterministic FP64 reduction ------------------------------------------------
@@ -590,7 +594,10 @@ __device__ __forceinline__ void slot_flush(double (*lds)[4][4], double* partials
 }
 
 // ---- the tile loop ------------------------------------------------------------------
-template <int FORM, int CPT>
+// SOLO: a single-tunnel launch (k_step, k_step_adv), which stores the macroscopic
+// field and loads the c_x = 0 planes non-temporally; batched launches keep both
+// cached (measured: sessions 0.5-1 % faster each way, batches 0.35-0.8 % slower).
+template <int FORM, int CPT, bool SOLO>
 __device__ __forceinline__ void run_tile(const StepDesc& d, int parity, uint32_t tile, int flags) {
   typedef Form<FORM> S;
   typedef typename S::real R;
@@ -606,7 +613,7 @@ __device__ __forceinline__ void run_tile(const StepDesc& d, int parity, uint32_t
   for (int k = 0; k < CPT; k++) {
     const uint32_t idx = base + (uint32_t)(k * BLOCK) + threadIdx.x;
     Acc a{0.0, 0.0, 0.0, 0.0};
-    if (idx < d.n) step_cell<FORM>(d, fin, fout, shin, shout, idx, flags, a);
+    if (idx < d.n) step_cell<FORM, SOLO>(d, fin, fout, shin, shout, idx, flags, a);
     if (flags & (WT_FLAG_FORCES | WT_FLAG_MASS)) slot_stage(lds, k, a);
   }
   if (flags & (WT_FLAG_FORCES | WT_FLAG_MASS))
@@ -631,7 +638,7 @@ __device__ __forceinline__ uint32_t xcd_tile(uint32_t b, uint32_t nblocks, int r
 template <int FORM, int CPT>
 __global__ __launch_bounds__(BLOCK) WT_STEP_ATTR void k_step(StepDesc d, int parity, int remap, int flags) {
   const uint32_t tile = xcd_tile(blockIdx.x, gridDim.x, remap);
-  run_tile<FORM, CPT>(d, parity, tile, flags);
+  run_tile<FORM, CPT, true>(d, parity, tile, flags);
 }
 
 // One lattice step plus the particle advection of the previous step from its
@@ -646,7 +653,7 @@ __global__ __launch_bounds__(BLOCK) WT_STEP_ATTR void k_step_adv(StepDesc d, Adv
                                                                  int flags, uint32_t step_blocks) {
   const uint32_t nadv = gridDim.x - step_blocks;
   if (blockIdx.x >= nadv) {
-    run_tile<FORM, CPT>(d, parity, xcd_tile(blockIdx.x - nadv, step_blocks, remap), flags);
+    run_tile<FORM, CPT, true>(d, parity, xcd_tile(blockIdx.x - nadv, step_blocks, remap), flags);
   } else {
     const uint32_t j = blockIdx.x * BLOCK + threadIdx.x;
     if (j < a.num_p) advect_particle(a, j);
@@ -663,7 +670,7 @@ __global__ __launch_bounds__(BLOCK) void k_step_batch(const StepDesc* __restrict
     if (ds[mid].tile_begin <= tile) lo = mid; else hi = mid - 1;
   }
   const StepDesc d = ds[lo];
-  run_tile<FORM, CPT>(d, parity ^ (d.flags & 1), tile - d.tile_begin, flags);
+  run_tile<FORM, CPT, false>(d, parity ^ (d.flags & 1), tile - d.tile_begin, flags);
 }
 
 #ifndef WT_STEP_FP32_UNIT  // kernels launched only from the main unit
@@ -1254,14 +1261,13 @@ int wt_batch_step(void* workspace, const wt_batch_info* info, int parity, int fl
   const StepDesc* dd = (const StepDesc*)workspace;
   const RedDesc* rd = (const RedDesc*)((const char*)workspace + sizeof(StepDesc) * nt);
   const int f = flags & (WT_FLAG_FORCES | WT_FLAG_MASS);
-  const int fk = f | FLAG_MAC_CACHED;
   const uint32_t ntile = (uint32_t)info->total_tiles;
   const int remap = (info->remap & 1) | (((info->remap & 2) && (parity & 1)) ? 2 : 0);
   switch (form) {
-    case WT_FP32: launch_batch_f<WT_FP32>(dd, nt, ntile, parity, info->cpt, remap, fk, s); break;
-    case WT_FP16: launch_batch_f<WT_FP16>(dd, nt, ntile, parity, info->cpt, remap, fk, s); break;
-    case WT_BF16: launch_batch_f<WT_BF16>(dd, nt, ntile, parity, info->cpt, remap, fk, s); break;
-    case WT_FP64: launch_batch_f<WT_FP64>(dd, nt, ntile, parity, info->cpt, remap, fk, s); break;
+    case WT_FP32: launch_batch_f<WT_FP32>(dd, nt, ntile, parity, info->cpt, remap, f, s); break;
+    case WT_FP16: launch_batch_f<WT_FP16>(dd, nt, ntile, parity, info->cpt, remap, f, s); break;
+    case WT_BF16: launch_batch_f<WT_BF16>(dd, nt, ntile, parity, info->cpt, remap, f, s); break;
+    case WT_FP64: launch_batch_f<WT_FP64>(dd, nt, ntile, parity, info->cpt, remap, f, s); break;
   }
   if (f) {
     hipLaunchKernelGGL(k_reduce1, dim3((unsigned)info->max_chunks, (unsigned)nt), dim3(BLOCK), 0, s, rd);

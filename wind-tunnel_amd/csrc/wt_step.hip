@@ -596,7 +596,8 @@ __device__ __forceinline__ void slot_flush(double (*lds)[4][4], double* partials
 // ---- the tile loop ------------------------------------------------------------------
 // SOLO: a single-tunnel launch (k_step, k_step_adv), which stores the macroscopic
 // field and loads the c_x = 0 planes non-temporally; batched launches keep both
-// cached (measured: sessions 0.5-1 % faster each way, batches 0.35-0.8 % slower).
+// cached (measured at 33.5M cells: session steps 0.9-1.05 % faster for FP32 and
+// 0.7-1.2 % for FP16 with both; the 48-tunnel batch 0.35 and 0.8 % slower).
 template <int FORM, int CPT, bool SOLO>
 __device__ __forceinline__ void run_tile(const StepDesc& d, int parity, uint32_t tile, int flags) {
   typedef Form<FORM> S;

@@ -39,6 +39,13 @@ BYTES_PER_CELL = {'fp32': 169, 'fp16': 93, 'bf16': 93, 'fp64': 321}
 BYTES_PER_PARTICLE = 40
 EVENT_EVERY, EVENT_PHASE = 10, 5     # HIP events around steps numbered 5 mod 10
 FORM_IDS = {'fp32': 0, 'fp16': 1, 'bf16': 2, 'fp64': 3}
+ELEM_BYTES = {'fp32': 4, 'fp16': 2, 'bf16': 2, 'fp64': 8}
+
+
+def streaming(form, n):
+    """Whether the library runs the step's streaming cache policy on this tunnel:
+    both population buffers above 2 GiB (the third template argument of k_step)."""
+    return 2 * 19 * n * ELEM_BYTES[form] >= 1 << 31
 DIMS = (512, 256, 256)
 UIN, TAU, SMAG = 0.05, 0.51, 0.16
 BODY_SEED = 2025
@@ -99,8 +106,9 @@ def run_gpu(form, steps, warmup, mask, counts, lref):
         'ms_per_step': wall * 1e3 / steps, 'step_kernel_ms': step_ms,
         'kernel_launches_timed': len(plain),
         'kernel_mlups': n / (step_ms * 1e-3) / 1e6, 'fused_advect': s.fuse, 'bytes_per_launch': nbytes,
-        'kernel': f"k_step_adv<{FORM_IDS[form]}, 1> (step + previous step's advection)" if s.fuse
-                  else f'k_step<{FORM_IDS[form]}, 1> (fused stream/collide)',
+        'kernel': (f"k_step_adv<{FORM_IDS[form]}, 1, {str(streaming(form, n)).lower()}> "
+                   "(step + previous step's advection)") if s.fuse
+                  else f'k_step<{FORM_IDS[form]}, 1, {str(streaming(form, n)).lower()}> (fused stream/collide)',
         'achieved_gbs': nbytes / (step_ms * 1e-3) / 1e9,
         'cd': float(cd), 'cl': float(cl), 'viewer_cd_cl': None if viewer is None else [float(v) for v in viewer],
     }

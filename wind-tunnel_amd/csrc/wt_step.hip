@@ -636,10 +636,10 @@ __device__ __forceinline__ uint32_t xcd_tile(uint32_t b, uint32_t nblocks, int r
   return (remap & 2) ? nblocks - 1u - t : t;
 }
 
-template <int FORM, int CPT>
+template <int FORM, int CPT, bool SOLO>
 __global__ __launch_bounds__(BLOCK) WT_STEP_ATTR void k_step(StepDesc d, int parity, int remap, int flags) {
   const uint32_t tile = xcd_tile(blockIdx.x, gridDim.x, remap);
-  run_tile<FORM, CPT, true>(d, parity, tile, flags);
+  run_tile<FORM, CPT, SOLO>(d, parity, tile, flags);
 }
 
 // One lattice step plus the particle advection of the previous step from its
@@ -649,12 +649,12 @@ __global__ __launch_bounds__(BLOCK) WT_STEP_ATTR void k_step(StepDesc d, int par
 // follow in the tile order of k_step. (Measured at 33.5M cells and 2^20
 // particles: advection first costs ~7 us over the plain step, spread evenly
 // through the tiles ~14 us, after the tiles ~19 us, as much as its own launch.)
-template <int FORM, int CPT>
+template <int FORM, int CPT, bool SOLO>
 __global__ __launch_bounds__(BLOCK) WT_STEP_ATTR void k_step_adv(StepDesc d, AdvectArgs a, int parity, int remap,
                                                                  int flags, uint32_t step_blocks) {
   const uint32_t nadv = gridDim.x - step_blocks;
   if (blockIdx.x >= nadv) {
-    run_tile<FORM, CPT, true>(d, parity, xcd_tile(blockIdx.x - nadv, step_blocks, remap), flags);
+    run_tile<FORM, CPT, SOLO>(d, parity, xcd_tile(blockIdx.x - nadv, step_blocks, remap), flags);
   } else {
     const uint32_t j = blockIdx.x * BLOCK + threadIdx.x;
     if (j < a.num_p) advect_particle(a, j);
@@ -893,35 +893,40 @@ __global__ __launch_bounds__(BLOCK) void k_link_classes(const uint8_t* __restric
 struct Shape {
   int cpt;
   int remap;
+  int solo;   // 1: the streaming cache policy of large single tunnels (run_tile's SOLO)
 };
 
 template <int FORM, int CPT>
-static void launch_step_t(const StepDesc& d, int parity, int remap, int flags, hipStream_t s) {
+static void launch_step_t(const StepDesc& d, int parity, int remap, int flags, int solo, hipStream_t s) {
   const uint32_t ntile = (d.n + BLOCK * CPT - 1) / (BLOCK * CPT);
-  hipLaunchKernelGGL((k_step<FORM, CPT>), dim3(ntile), dim3(BLOCK), 0, s, d, parity, remap, flags);
+  if (solo) hipLaunchKernelGGL((k_step<FORM, CPT, true>), dim3(ntile), dim3(BLOCK), 0, s, d, parity, remap, flags);
+  else hipLaunchKernelGGL((k_step<FORM, CPT, false>), dim3(ntile), dim3(BLOCK), 0, s, d, parity, remap, flags);
 }
 
 template <int FORM>
 void launch_step_f(const StepDesc& d, int parity, Shape sh, int flags, hipStream_t s) {
   switch (sh.cpt) {
-    case 2: launch_step_t<FORM, 2>(d, parity, sh.remap, flags, s); break;
-    case 4: launch_step_t<FORM, 4>(d, parity, sh.remap, flags, s); break;
-    default: launch_step_t<FORM, 1>(d, parity, sh.remap, flags, s); break;
+    case 2: launch_step_t<FORM, 2>(d, parity, sh.remap, flags, sh.solo, s); break;
+    case 4: launch_step_t<FORM, 4>(d, parity, sh.remap, flags, sh.solo, s); break;
+    default: launch_step_t<FORM, 1>(d, parity, sh.remap, flags, sh.solo, s); break;
   }
 }
 
 template <int FORM, int CPT>
-static void launch_step_adv_t(const StepDesc& d, const AdvectArgs& a, int remap, int flags, hipStream_t s) {
+static void launch_step_adv_t(const StepDesc& d, const AdvectArgs& a, int remap, int flags, int solo, hipStream_t s) {
   const uint32_t ntile = (d.n + BLOCK * CPT - 1) / (BLOCK * CPT);
   const uint32_t nadv = (a.num_p + BLOCK - 1) / BLOCK;
-  hipLaunchKernelGGL((k_step_adv<FORM, CPT>), dim3(ntile + nadv), dim3(BLOCK), 0, s, d, a, 0, remap, flags, ntile);
+  if (solo)
+    hipLaunchKernelGGL((k_step_adv<FORM, CPT, true>), dim3(ntile + nadv), dim3(BLOCK), 0, s, d, a, 0, remap, flags, ntile);
+  else
+    hipLaunchKernelGGL((k_step_adv<FORM, CPT, false>), dim3(ntile + nadv), dim3(BLOCK), 0, s, d, a, 0, remap, flags, ntile);
 }
 template <int FORM>
 void launch_step_adv_f(const StepDesc& d, const AdvectArgs& a, Shape sh, int flags, hipStream_t s) {
   switch (sh.cpt) {
-    case 2: launch_step_adv_t<FORM, 2>(d, a, sh.remap, flags, s); break;
-    case 4: launch_step_adv_t<FORM, 4>(d, a, sh.remap, flags, s); break;
-    default: launch_step_adv_t<FORM, 1>(d, a, sh.remap, flags, s); break;
+    case 2: launch_step_adv_t<FORM, 2>(d, a, sh.remap, flags, sh.solo, s); break;
+    case 4: launch_step_adv_t<FORM, 4>(d, a, sh.remap, flags, sh.solo, s); break;
+    default: launch_step_adv_t<FORM, 1>(d, a, sh.remap, flags, sh.solo, s); break;
   }
 }
 
@@ -1012,7 +1017,17 @@ static int reduce_one(const wt_grid* g, hipStream_t s) {
 }
 
 
-static Shape g_shape[4] = {{1, 0}, {1, 0}, {1, 0}, {1, 0}};
+static Shape g_shape[4] = {{1, 0, 0}, {1, 0, 0}, {1, 0, 0}, {1, 0, 0}};
+
+// The streaming cache policy (run_tile's SOLO) pays only where the tunnel is far
+// larger than the 256 MB Infinity Cache: with both population buffers above 2 GiB
+// (FP32 from ~14M cells, FP16 from ~28M). Measured against cached accesses, single
+// tunnels: 33.5M cells 0.9-1.05 % (FP32) and 0.7-1.2 % (FP16) faster; 11.8M cells
+// even (FP32) and 1.5 % slower (FP16); 3.9M cells 5.8 % / 8.4 % slower; 0.5M 2 %.
+static int solo_policy(const wt_grid* g) {
+  const int64_t n = (int64_t)g->nx * g->ny * g->nz;
+  return 2 * (int64_t)Q * n * form_elem_bytes(g->form) >= ((int64_t)1 << 31) ? 1 : 0;
+}
 
 static void launch_step(int form, const StepDesc& d, int parity, Shape sh, int flags, hipStream_t s) {
   switch (form) {
@@ -1111,6 +1126,7 @@ int wt_step(const wt_grid* g, int cur, int nsteps, const wt_params* p, int flags
     return fail(-1, "reduction flags need partials and sums buffers");
   hipStream_t s = (hipStream_t)stream;
   const Shape sh = g_shape[g->form];
+  const int solo = solo_policy(g);
   StepDesc d = make_desc(g, cur, p, 0);
   if (sh.remap & 4) d.global_addr = 1;  // shape bit 2: 64-bit global addressing at any size
   for (int k = 0; k < nsteps; k++) {
@@ -1118,6 +1134,7 @@ int wt_step(const wt_grid* g, int cur, int nsteps, const wt_params* p, int flags
     // with shape bit 1 the sweep direction alternates with the buffer parity
     Shape shk = sh;
     shk.remap = (sh.remap & 1) | (((sh.remap & 2) && ((cur ^ k) & 1)) ? 2 : 0);
+    shk.solo = solo;
     launch_step(g->form, d, k & 1, shk, f, s);
   }
   if (nsteps > 0 && (flags & (WT_FLAG_FORCES | WT_FLAG_MASS))) {
@@ -1141,6 +1158,7 @@ int wt_step_advect(const wt_grid* g, int cur, const wt_params* p, int flags, con
   hipStream_t s = (hipStream_t)stream;
   Shape sh = g_shape[g->form];
   sh.remap = (sh.remap & 1) | (((sh.remap & 2) && (cur & 1)) ? 2 : 0);   // as step k = 0 of wt_step
+  sh.solo = solo_policy(g);
   StepDesc d = make_desc(g, cur, p, 0);
   if (g_shape[g->form].remap & 4) d.global_addr = 1;
   const wt_seed_box none{0.f, 0.f, 0.f, 0.f, 0.f};

@@ -65,9 +65,11 @@ template <typename T> __device__ __forceinline__ T maxv(T a, T b) { return (a < 
 // lines one wave reads whole, 0.9 % (FP32) and 0.5 % (FP16) faster per session
 // step. Storing the macroscopic field non-temporally made session steps (step,
 // particle advection and slices, which read the field) 0.8 % faster for FP16 and
-// left FP32 the same. Batched steps lose with both (0.35 and 0.8 %), so only
-// single-tunnel launches use them (run_tile's SOLO), on the raw-buffer paths; the
-// 64-bit global-address path loads every plane cached.
+// left FP32 the same. Batched steps lose with both (0.35 and 0.8 %), and so do
+// single tunnels whose state partly fits the Infinity Cache, so only launches of
+// one tunnel with more than 2 GiB of populations use them (run_tile's SOLO,
+// solo_policy), on the raw-buffer paths; the 64-bit global-address path loads
+// every plane cached.
 #ifndef WT_LOAD_NT
 #define WT_LOAD_NT 0
 #endif
@@ -594,10 +596,11 @@ __device__ __forceinline__ void slot_flush(double (*lds)[4][4], double* partials
 }
 
 // ---- the tile loop ------------------------------------------------------------------
-// SOLO: a single-tunnel launch (k_step, k_step_adv), which stores the macroscopic
-// field and loads the c_x = 0 planes non-temporally; batched launches keep both
-// cached (measured at 33.5M cells: session steps 0.9-1.05 % faster for FP32 and
-// 0.7-1.2 % for FP16 with both; the 48-tunnel batch 0.35 and 0.8 % slower).
+// SOLO: a launch of one large tunnel (k_step, k_step_adv where solo_policy holds),
+// which stores the macroscopic field and loads the c_x = 0 planes non-temporally;
+// smaller tunnels and batched launches keep both cached (measured at 33.5M cells:
+// session steps 0.9-1.05 % faster for FP32 and 0.7-1.2 % for FP16 with both; the
+// 48-tunnel batch 0.35 and 0.8 % slower).
 template <int FORM, int CPT, bool SOLO>
 __device__ __forceinline__ void run_tile(const StepDesc& d, int parity, uint32_t tile, int flags) {
   typedef Form<FORM> S;

@@ -42,10 +42,12 @@ FORM_IDS = {'fp32': 0, 'fp16': 1, 'bf16': 2, 'fp64': 3}
 ELEM_BYTES = {'fp32': 4, 'fp16': 2, 'bf16': 2, 'fp64': 8}
 
 
-def streaming(form, n):
-    """Whether the library runs the step's streaming cache policy on this tunnel:
-    both population buffers above 2 GiB (the third template argument of k_step)."""
-    return 2 * 19 * n * ELEM_BYTES[form] >= 1 << 31
+def cache_policy(form, n):
+    """The cache policy the library's single-tunnel step runs on this tunnel (the
+    third template argument of k_step), by the bytes of both population buffers:
+    1 streaming above 2 GiB, 2 all cached within 128 MiB, else 0."""
+    b = 2 * 19 * n * ELEM_BYTES[form]
+    return 1 if b >= 1 << 31 else (2 if b <= 128 << 20 else 0)
 DIMS = (512, 256, 256)
 UIN, TAU, SMAG = 0.05, 0.51, 0.16
 BODY_SEED = 2025
@@ -106,9 +108,9 @@ def run_gpu(form, steps, warmup, mask, counts, lref):
         'ms_per_step': wall * 1e3 / steps, 'step_kernel_ms': step_ms,
         'kernel_launches_timed': len(plain),
         'kernel_mlups': n / (step_ms * 1e-3) / 1e6, 'fused_advect': s.fuse, 'bytes_per_launch': nbytes,
-        'kernel': (f"k_step_adv<{FORM_IDS[form]}, 1, {str(streaming(form, n)).lower()}> "
+        'kernel': (f"k_step_adv<{FORM_IDS[form]}, 1, {cache_policy(form, n)}> "
                    "(step + previous step's advection)") if s.fuse
-                  else f'k_step<{FORM_IDS[form]}, 1, {str(streaming(form, n)).lower()}> (fused stream/collide)',
+                  else f'k_step<{FORM_IDS[form]}, 1, {cache_policy(form, n)}> (fused stream/collide)',
         'achieved_gbs': nbytes / (step_ms * 1e-3) / 1e9,
         'cd': float(cd), 'cl': float(cl), 'viewer_cd_cl': None if viewer is None else [float(v) for v in viewer],
     }

@@ -67,9 +67,10 @@ template <typename T> __device__ __forceinline__ T maxv(T a, T b) { return (a < 
 // particle advection and slices, which read the field) 0.8 % faster for FP16 and
 // left FP32 the same. Batched steps lose with both (0.35 and 0.8 %), and so do
 // single tunnels whose state partly fits the Infinity Cache, so only launches of
-// one tunnel with more than 2 GiB of populations use them (run_tile's SOLO,
+// one tunnel with more than 2 GiB of populations use them (run_tile's POL 1,
 // solo_policy), on the raw-buffer paths; the 64-bit global-address path loads
-// every plane cached.
+// every plane cached. A tunnel whose populations fit in 128 MiB stores them cached
+// too (POL 2).
 #ifndef WT_LOAD_NT
 #define WT_LOAD_NT 0
 #endif
@@ -80,8 +81,8 @@ template <typename E> __device__ __forceinline__ E ld_stream(const E* p) {
   if constexpr (WT_LOAD_NT) return __builtin_nontemporal_load(p);
   else return *p;
 }
-template <typename E> __device__ __forceinline__ void st_stream(E* p, E v) {
-  if constexpr (WT_STORE_NT) __builtin_nontemporal_store(v, p);
+template <bool NT, typename E> __device__ __forceinline__ void st_stream(E* p, E v) {
+  if constexpr (NT && WT_STORE_NT) __builtin_nontemporal_store(v, p);
   else *p = v;
 }
 
@@ -341,8 +342,9 @@ template <> struct RawIO<float> {
     return __uint_as_float(nt ? __builtin_amdgcn_raw_buffer_load_b32(r, vo, so, CPOL_LD_NT)
                               : __builtin_amdgcn_raw_buffer_load_b32(r, vo, so, CPOL_LD));
   }
-  __device__ static void st(__amdgpu_buffer_rsrc_t r, uint32_t vo, uint32_t so, float v) {
-    __builtin_amdgcn_raw_buffer_store_b32(__float_as_uint(v), r, vo, so, CPOL_ST);
+  __device__ static void st(__amdgpu_buffer_rsrc_t r, uint32_t vo, uint32_t so, float v, bool nt) {
+    if (nt) __builtin_amdgcn_raw_buffer_store_b32(__float_as_uint(v), r, vo, so, CPOL_ST);
+    else __builtin_amdgcn_raw_buffer_store_b32(__float_as_uint(v), r, vo, so, 0);
   }
 };
 template <> struct RawIO<uint16_t> {
@@ -350,13 +352,14 @@ template <> struct RawIO<uint16_t> {
     return nt ? __builtin_amdgcn_raw_buffer_load_b16(r, vo, so, CPOL_LD_NT)
               : __builtin_amdgcn_raw_buffer_load_b16(r, vo, so, CPOL_LD);
   }
-  __device__ static void st(__amdgpu_buffer_rsrc_t r, uint32_t vo, uint32_t so, uint16_t v) {
-    __builtin_amdgcn_raw_buffer_store_b16(v, r, vo, so, CPOL_ST);
+  __device__ static void st(__amdgpu_buffer_rsrc_t r, uint32_t vo, uint32_t so, uint16_t v, bool nt) {
+    if (nt) __builtin_amdgcn_raw_buffer_store_b16(v, r, vo, so, CPOL_ST);
+    else __builtin_amdgcn_raw_buffer_store_b16(v, r, vo, so, 0);
   }
 };
 
 // ---- one cell, general path (any mix of solid, face and near-wall lanes) ------------
-template <int FORM, bool SOLO>
+template <int FORM, int POL>
 __device__ __forceinline__ void step_cell_general(const StepDesc& d, const typename Form<FORM>::elem* fin,
                                                   typename Form<FORM>::elem* fout,
                                                   const Shell<typename Form<FORM>::real>& shin,
@@ -371,7 +374,7 @@ __device__ __forceinline__ void step_cell_general(const StepDesc& d, const typen
   const R zero = (R)0, one = (R)1;
 
   if ((c & CLS_CODE) != 0) {  // solid cell, lbm.py:81-83
-    st_mac<SOLO>(d.mac + idx, 0.f, 0.f, 0.f, 1.f);
+    st_mac<POL == 1>(d.mac + idx, 0.f, 0.f, 0.f, 1.f);
     shell_write<R>(shout, x, y, z, zero, zero, zero, one);
     return;
   }
@@ -382,14 +385,14 @@ __device__ __forceinline__ void step_cell_general(const StepDesc& d, const typen
   auto store_all = [&](const E (&v)[Q]) {
     auto global_stores = [&]() {
 #pragma unroll
-      for (int i = 0; i < Q; i++) st_stream(fout + (pc + (uint32_t)ppos(i) * (uint32_t)nx), v[i]);
+      for (int i = 0; i < Q; i++) st_stream<POL != 2>(fout + (pc + (uint32_t)ppos(i) * (uint32_t)nx), v[i]);
     };
     if constexpr (sizeof(E) <= 4) {
       if (!d.global_addr) {
         const __amdgpu_buffer_rsrc_t rout =
             __builtin_amdgcn_make_buffer_rsrc((void*)fout, (short)0, (int)((uint32_t)Q * n * ES), BUF_GFX9_WORD3);
 #pragma unroll
-        for (int i = 0; i < Q; i++) RawIO<E>::st(rout, bo, (uint32_t)ppos(i) * (uint32_t)nx * ES, v[i]);
+        for (int i = 0; i < Q; i++) RawIO<E>::st(rout, bo, (uint32_t)ppos(i) * (uint32_t)nx * ES, v[i], POL != 2);
       } else {
         global_stores();
       }
@@ -405,7 +408,7 @@ __device__ __forceinline__ void step_cell_general(const StepDesc& d, const typen
 #pragma unroll
     for (int i = 0; i < Q; i++) v[i] = S::store(face_feq<R>(i, br, bux, buy, buz, bsq), i);
     store_all(v);
-    st_mac<SOLO>(d.mac + idx, (float)bux, (float)buy, (float)buz, (float)br);
+    st_mac<POL == 1>(d.mac + idx, (float)bux, (float)buy, (float)buz, (float)br);
     if (flags & WT_FLAG_MASS) acc.m += (double)br;
     return;
   }
@@ -440,7 +443,7 @@ __device__ __forceinline__ void step_cell_general(const StepDesc& d, const typen
         const uint32_t pull = pshift(i, nx, nxy) * ES + bias;
         const uint32_t wall = (uint32_t)ppos(opp(i)) * (uint32_t)nx * ES + bias;
         const uint32_t m = 0u - ((lk.bb >> i) & 1u);
-        f[i] = S::load(RawIO<E>::ld(rin, bo + (pull + (m & (wall - pull))), 0u, SOLO && cx(i) == 0), i);
+        f[i] = S::load(RawIO<E>::ld(rin, bo + (pull + (m & (wall - pull))), 0u, POL == 1 && cx(i) == 0), i);
       }
     } else {
       global_loads();
@@ -460,7 +463,7 @@ __device__ __forceinline__ void step_cell_general(const StepDesc& d, const typen
 #pragma unroll
   for (int i = 0; i < Q; i++) st[i] = S::store(out[i], i);
   store_all(st);
-  st_mac<SOLO>(d.mac + idx, (float)ux, (float)uy, (float)uz, (float)rho);
+  st_mac<POL == 1>(d.mac + idx, (float)ux, (float)uy, (float)uz, (float)rho);
   shell_write<R>(shout, x, y, z, ux, uy, uz, rho);
   if (flags & WT_FLAG_MASS) acc.m += (double)rho;
   if ((flags & WT_FLAG_FORCES) && (c & CLS_FORCE)) add_force<FORM>(d.cls, idx, nx, nxy, st, acc);
@@ -471,7 +474,7 @@ __device__ __forceinline__ void step_cell_general(const StepDesc& d, const typen
 // of population i is the lane's own element offset (one vector register shared
 // by all 38 loads and stores) plus the wave-uniform pshift(i) in a scalar
 // register (raw-buffer offsets, as above), with no per-lane address arithmetic.
-template <int FORM, bool SOLO>
+template <int FORM, int POL>
 __device__ __forceinline__ void step_cell_fast(const StepDesc& d, const typename Form<FORM>::elem* fin,
                                                typename Form<FORM>::elem* fout,
                                                const Shell<typename Form<FORM>::real>& shin,
@@ -508,7 +511,7 @@ __device__ __forceinline__ void step_cell_fast(const StepDesc& d, const typename
 #pragma unroll
         for (int k = 0; k < Q; k++) {
           const int i = LOAD_ORDER[k];
-          f[i] = S::load(RawIO<E>::ld(rin, bo, pshift(i, nx, nxy) * ES + bias, SOLO && cx(i) == 0), i);
+          f[i] = S::load(RawIO<E>::ld(rin, bo, pshift(i, nx, nxy) * ES + bias, POL == 1 && cx(i) == 0), i);
         }
       } else {
         global_loads();
@@ -520,26 +523,26 @@ __device__ __forceinline__ void step_cell_fast(const StepDesc& d, const typename
   }
   auto global_stores = [&]() {
 #pragma unroll
-    for (int i = 0; i < Q; i++) st_stream(fout + (pc + (uint32_t)ppos(i) * nx), S::store(out[i], i));
+    for (int i = 0; i < Q; i++) st_stream<POL != 2>(fout + (pc + (uint32_t)ppos(i) * nx), S::store(out[i], i));
   };
   if constexpr (sizeof(E) <= 4) {
     if (!d.global_addr) {
       const __amdgpu_buffer_rsrc_t rout =
           __builtin_amdgcn_make_buffer_rsrc((void*)fout, (short)0, (int)((uint32_t)Q * n * ES), BUF_GFX9_WORD3);
 #pragma unroll
-      for (int i = 0; i < Q; i++) RawIO<E>::st(rout, bo, (uint32_t)ppos(i) * nx * ES, S::store(out[i], i));
+      for (int i = 0; i < Q; i++) RawIO<E>::st(rout, bo, (uint32_t)ppos(i) * nx * ES, S::store(out[i], i), POL != 2);
     } else {
       global_stores();
     }
   } else {
     global_stores();
   }
-  st_mac<SOLO>(d.mac + idx, (float)ux, (float)uy, (float)uz, (float)rho);
+  st_mac<POL == 1>(d.mac + idx, (float)ux, (float)uy, (float)uz, (float)rho);
   if (!xface) shell_write<R>(shout, x, y, z, ux, uy, uz, rho);
   if (flags & WT_FLAG_MASS) acc.m += (double)rho;
 }
 
-template <int FORM, bool SOLO>
+template <int FORM, int POL>
 __device__ __forceinline__ void step_cell(const StepDesc& d, const typename Form<FORM>::elem* fin,
                                           typename Form<FORM>::elem* fout,
                                           const Shell<typename Form<FORM>::real>& shin,
@@ -558,9 +561,9 @@ __device__ __forceinline__ void step_cell(const StepDesc& d, const typename Form
   // fluid lanes away from walls, and fluid inlet/outlet lanes, take the fast path
   const bool simple = (c == 0) & (!face | xface) & (y != 0) & (y != ny - 1) & (z != 0) & (z != nz - 1);
   if (__all(simple))
-    step_cell_fast<FORM, SOLO>(d, fin, fout, shin, shout, idx, pc, x, y, z, xface, flags, acc);
+    step_cell_fast<FORM, POL>(d, fin, fout, shin, shout, idx, pc, x, y, z, xface, flags, acc);
   else
-    step_cell_general<FORM, SOLO>(d, fin, fout, shin, shout, idx, pc, c, x, y, z, face, flags, acc);
+    step_cell_general<FORM, POL>(d, fin, fout, shin, shout, idx, pc, c, x, y, z, face, flags, acc);
 }
 
 // ---- deterministic FP64 reduction ------------------------------------------------
@@ -596,12 +599,15 @@ __device__ __forceinline__ void slot_flush(double (*lds)[4][4], double* partials
 }
 
 // ---- the tile loop ------------------------------------------------------------------
-// SOLO: a launch of one large tunnel (k_step, k_step_adv where solo_policy holds),
-// which stores the macroscopic field and loads the c_x = 0 planes non-temporally;
-// smaller tunnels and batched launches keep both cached (measured at 33.5M cells:
-// session steps 0.9-1.05 % faster for FP32 and 0.7-1.2 % for FP16 with both; the
-// 48-tunnel batch 0.35 and 0.8 % slower).
-template <int FORM, int CPT, bool SOLO>
+// POL, the cache policy of a launch (chosen by cache_policy on the host):
+//   0  batches and mid-size tunnels: populations stored non-temporally, the rest cached;
+//   1  one tunnel with more than 2 GiB of populations: also the macroscopic field
+//      stored and the c_x = 0 planes loaded non-temporally (33.5M cells: session
+//      steps 0.9-1.05 % faster for FP32, 0.7-1.2 % for FP16; batches and smaller
+//      tunnels lose with it);
+//   2  one tunnel whose two population buffers fit in 128 MiB: everything cached,
+//      so the Infinity Cache carries a step's output into the next step.
+template <int FORM, int CPT, int POL>
 __device__ __forceinline__ void run_tile(const StepDesc& d, int parity, uint32_t tile, int flags) {
   typedef Form<FORM> S;
   typedef typename S::real R;
@@ -617,7 +623,7 @@ __device__ __forceinline__ void run_tile(const StepDesc& d, int parity, uint32_t
   for (int k = 0; k < CPT; k++) {
     const uint32_t idx = base + (uint32_t)(k * BLOCK) + threadIdx.x;
     Acc a{0.0, 0.0, 0.0, 0.0};
-    if (idx < d.n) step_cell<FORM, SOLO>(d, fin, fout, shin, shout, idx, flags, a);
+    if (idx < d.n) step_cell<FORM, POL>(d, fin, fout, shin, shout, idx, flags, a);
     if (flags & (WT_FLAG_FORCES | WT_FLAG_MASS)) slot_stage(lds, k, a);
   }
   if (flags & (WT_FLAG_FORCES | WT_FLAG_MASS))
@@ -639,10 +645,10 @@ __device__ __forceinline__ uint32_t xcd_tile(uint32_t b, uint32_t nblocks, int r
   return (remap & 2) ? nblocks - 1u - t : t;
 }
 
-template <int FORM, int CPT, bool SOLO>
+template <int FORM, int CPT, int POL>
 __global__ __launch_bounds__(BLOCK) WT_STEP_ATTR void k_step(StepDesc d, int parity, int remap, int flags) {
   const uint32_t tile = xcd_tile(blockIdx.x, gridDim.x, remap);
-  run_tile<FORM, CPT, SOLO>(d, parity, tile, flags);
+  run_tile<FORM, CPT, POL>(d, parity, tile, flags);
 }
 
 // One lattice step plus the particle advection of the previous step from its
@@ -652,12 +658,12 @@ __global__ __launch_bounds__(BLOCK) WT_STEP_ATTR void k_step(StepDesc d, int par
 // follow in the tile order of k_step. (Measured at 33.5M cells and 2^20
 // particles: advection first costs ~7 us over the plain step, spread evenly
 // through the tiles ~14 us, after the tiles ~19 us, as much as its own launch.)
-template <int FORM, int CPT, bool SOLO>
+template <int FORM, int CPT, int POL>
 __global__ __launch_bounds__(BLOCK) WT_STEP_ATTR void k_step_adv(StepDesc d, AdvectArgs a, int parity, int remap,
                                                                  int flags, uint32_t step_blocks) {
   const uint32_t nadv = gridDim.x - step_blocks;
   if (blockIdx.x >= nadv) {
-    run_tile<FORM, CPT, SOLO>(d, parity, xcd_tile(blockIdx.x - nadv, step_blocks, remap), flags);
+    run_tile<FORM, CPT, POL>(d, parity, xcd_tile(blockIdx.x - nadv, step_blocks, remap), flags);
   } else {
     const uint32_t j = blockIdx.x * BLOCK + threadIdx.x;
     if (j < a.num_p) advect_particle(a, j);
@@ -674,7 +680,7 @@ __global__ __launch_bounds__(BLOCK) void k_step_batch(const StepDesc* __restrict
     if (ds[mid].tile_begin <= tile) lo = mid; else hi = mid - 1;
   }
   const StepDesc d = ds[lo];
-  run_tile<FORM, CPT, false>(d, parity ^ (d.flags & 1), tile - d.tile_begin, flags);
+  run_tile<FORM, CPT, 0>(d, parity ^ (d.flags & 1), tile - d.tile_begin, flags);
 }
 
 #ifndef WT_STEP_FP32_UNIT  // kernels launched only from the main unit
@@ -896,14 +902,17 @@ __global__ __launch_bounds__(BLOCK) void k_link_classes(const uint8_t* __restric
 struct Shape {
   int cpt;
   int remap;
-  int solo;   // 1: the streaming cache policy of large single tunnels (run_tile's SOLO)
+  int solo;   // the cache policy of the launch (run_tile's POL: 0, 1 or 2)
 };
 
 template <int FORM, int CPT>
 static void launch_step_t(const StepDesc& d, int parity, int remap, int flags, int solo, hipStream_t s) {
   const uint32_t ntile = (d.n + BLOCK * CPT - 1) / (BLOCK * CPT);
-  if (solo) hipLaunchKernelGGL((k_step<FORM, CPT, true>), dim3(ntile), dim3(BLOCK), 0, s, d, parity, remap, flags);
-  else hipLaunchKernelGGL((k_step<FORM, CPT, false>), dim3(ntile), dim3(BLOCK), 0, s, d, parity, remap, flags);
+  switch (solo) {
+    case 1: hipLaunchKernelGGL((k_step<FORM, CPT, 1>), dim3(ntile), dim3(BLOCK), 0, s, d, parity, remap, flags); break;
+    case 2: hipLaunchKernelGGL((k_step<FORM, CPT, 2>), dim3(ntile), dim3(BLOCK), 0, s, d, parity, remap, flags); break;
+    default: hipLaunchKernelGGL((k_step<FORM, CPT, 0>), dim3(ntile), dim3(BLOCK), 0, s, d, parity, remap, flags); break;
+  }
 }
 
 template <int FORM>
@@ -919,10 +928,11 @@ template <int FORM, int CPT>
 static void launch_step_adv_t(const StepDesc& d, const AdvectArgs& a, int remap, int flags, int solo, hipStream_t s) {
   const uint32_t ntile = (d.n + BLOCK * CPT - 1) / (BLOCK * CPT);
   const uint32_t nadv = (a.num_p + BLOCK - 1) / BLOCK;
-  if (solo)
-    hipLaunchKernelGGL((k_step_adv<FORM, CPT, true>), dim3(ntile + nadv), dim3(BLOCK), 0, s, d, a, 0, remap, flags, ntile);
-  else
-    hipLaunchKernelGGL((k_step_adv<FORM, CPT, false>), dim3(ntile + nadv), dim3(BLOCK), 0, s, d, a, 0, remap, flags, ntile);
+  switch (solo) {
+    case 1: hipLaunchKernelGGL((k_step_adv<FORM, CPT, 1>), dim3(ntile + nadv), dim3(BLOCK), 0, s, d, a, 0, remap, flags, ntile); break;
+    case 2: hipLaunchKernelGGL((k_step_adv<FORM, CPT, 2>), dim3(ntile + nadv), dim3(BLOCK), 0, s, d, a, 0, remap, flags, ntile); break;
+    default: hipLaunchKernelGGL((k_step_adv<FORM, CPT, 0>), dim3(ntile + nadv), dim3(BLOCK), 0, s, d, a, 0, remap, flags, ntile); break;
+  }
 }
 template <int FORM>
 void launch_step_adv_f(const StepDesc& d, const AdvectArgs& a, Shape sh, int flags, hipStream_t s) {
@@ -1022,14 +1032,18 @@ static int reduce_one(const wt_grid* g, hipStream_t s) {
 
 static Shape g_shape[4] = {{1, 0, 0}, {1, 0, 0}, {1, 0, 0}, {1, 0, 0}};
 
-// The streaming cache policy (run_tile's SOLO) pays only where the tunnel is far
-// larger than the 256 MB Infinity Cache: with both population buffers above 2 GiB
-// (FP32 from ~14M cells, FP16 from ~28M). Measured against cached accesses, single
-// tunnels: 33.5M cells 0.9-1.05 % (FP32) and 0.7-1.2 % (FP16) faster; 11.8M cells
-// even (FP32) and 1.5 % slower (FP16); 3.9M cells 5.8 % / 8.4 % slower; 0.5M 2 %.
+// Cache policy of a single-tunnel launch (run_tile's POL) by the bytes of its two
+// population buffers. Streaming (1) pays only far above the 256 MB Infinity Cache:
+// from 2 GiB (FP32 ~14M cells, FP16 ~28M); against policy 0 it measured 33.5M cells
+// 0.9-1.05 % (FP32) / 0.7-1.2 % (FP16) faster, 11.8M even / 1.5 % slower, 3.9M
+// 5.8 % / 8.4 % slower, 0.5M 2 % slower. All-cached (2) pays where the populations
+// fit the cache: 0.5M cells 8.8 % (FP32) / 6.5 % (FP16) faster than policy 0,
+// 3.9M and 11.8M cells 4-16 % slower.
 static int solo_policy(const wt_grid* g) {
   const int64_t n = (int64_t)g->nx * g->ny * g->nz;
-  return 2 * (int64_t)Q * n * form_elem_bytes(g->form) >= ((int64_t)1 << 31) ? 1 : 0;
+  const int64_t bytes = 2 * (int64_t)Q * n * form_elem_bytes(g->form);
+  if (bytes >= ((int64_t)1 << 31)) return 1;
+  return bytes <= ((int64_t)128 << 20) ? 2 : 0;
 }
 
 static void launch_step(int form, const StepDesc& d, int parity, Shape sh, int flags, hipStream_t s) {

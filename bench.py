@@ -45,9 +45,9 @@ ELEM_BYTES = {'fp32': 4, 'fp16': 2, 'bf16': 2, 'fp64': 8}
 def cache_policy(form, n):
     """The cache policy the library's single-tunnel step runs on this tunnel (the
     third template argument of k_step), by the bytes of both population buffers:
-    1 streaming above 2 GiB, 2 all cached within 128 MiB, else 0."""
+    1 streaming above 2 GiB, 2 all cached within 224 MiB, else 0."""
     b = 2 * 19 * n * ELEM_BYTES[form]
-    return 1 if b >= 1 << 31 else (2 if b <= 128 << 20 else 0)
+    return 1 if b >= 1 << 31 else (2 if b <= 224 << 20 else 0)
 DIMS = (512, 256, 256)
 UIN, TAU, SMAG = 0.05, 0.51, 0.16
 BODY_SEED = 2025

@@ -69,7 +69,7 @@ template <typename T> __device__ __forceinline__ T maxv(T a, T b) { return (a < 
 // single tunnels whose state partly fits the Infinity Cache, so only launches of
 // one tunnel with more than 2 GiB of populations use them (run_tile's POL 1,
 // solo_policy), on the raw-buffer paths; the 64-bit global-address path loads
-// every plane cached. A tunnel whose populations fit in 128 MiB stores them cached
+// every plane cached. A tunnel whose populations fit in 224 MiB stores them cached
 // too (POL 2).
 #ifndef WT_LOAD_NT
 #define WT_LOAD_NT 0
@@ -605,7 +605,7 @@ __device__ __forceinline__ void slot_flush(double (*lds)[4][4], double* partials
 //      stored and the c_x = 0 planes loaded non-temporally (33.5M cells: session
 //      steps 0.9-1.05 % faster for FP32, 0.7-1.2 % for FP16; batches and smaller
 //      tunnels lose with it);
-//   2  one tunnel whose two population buffers fit in 128 MiB: everything cached,
+//   2  one tunnel whose two population buffers fit in 224 MiB: everything cached,
 //      so the Infinity Cache carries a step's output into the next step.
 template <int FORM, int CPT, int POL>
 __device__ __forceinline__ void run_tile(const StepDesc& d, int parity, uint32_t tile, int flags) {
@@ -1036,14 +1036,15 @@ static Shape g_shape[4] = {{1, 0, 0}, {1, 0, 0}, {1, 0, 0}, {1, 0, 0}};
 // population buffers. Streaming (1) pays only far above the 256 MB Infinity Cache:
 // from 2 GiB (FP32 ~14M cells, FP16 ~28M); against policy 0 it measured 33.5M cells
 // 0.9-1.05 % (FP32) / 0.7-1.2 % (FP16) faster, 11.8M even / 1.5 % slower, 3.9M
-// 5.8 % / 8.4 % slower, 0.5M 2 % slower. All-cached (2) pays where the populations
-// fit the cache: 0.5M cells 8.8 % (FP32) / 6.5 % (FP16) faster than policy 0,
-// 3.9M and 11.8M cells 4-16 % slower.
+// 5.8 % / 8.4 % slower, 0.5M 2 % slower. All-cached (2) pays up to 224 MiB of
+// populations: faster than policy 0 by 8.8 % (FP32) / 6.5 % (FP16) at 0.5M cells,
+// 7.9 % (FP32) at 1.0M, 8.5 % (FP32, 196 MiB) at 1.35M, 12.5 % (FP16, 128 MiB) at
+// 1.77M; slower by 3.1 % (FP32, 257 MiB) at 1.77M and 4-16 % at 3.9M-11.8M cells.
 static int solo_policy(const wt_grid* g) {
   const int64_t n = (int64_t)g->nx * g->ny * g->nz;
   const int64_t bytes = 2 * (int64_t)Q * n * form_elem_bytes(g->form);
   if (bytes >= ((int64_t)1 << 31)) return 1;
-  return bytes <= ((int64_t)128 << 20) ? 2 : 0;
+  return bytes <= ((int64_t)224 << 20) ? 2 : 0;
 }
 
 static void launch_step(int form, const StepDesc& d, int parity, Shape sh, int flags, hipStream_t s) {

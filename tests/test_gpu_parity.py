@@ -210,6 +210,33 @@ def _shape_invariance(form, cpt, remap):
         assert _raw_equal(a, b)
 
 
+# The step kernel's cache policy follows the size of the two population buffers:
+# 2 (all cached) within 224 MiB, 0 up to 2 GiB, 1 (streaming) above. The small grids
+# above run policy 2 and the 33.5M-cell tests policy 1; these sizes run policy 0.
+@pytest.mark.parametrize('name,form,dims', [('fp32', O.FP32, (200, 96, 96)), ('fp16', O.FP16, (256, 112, 136))])
+def test_mid_size_tunnel_bit_exact_with_oracle(gpu, name, form, dims):
+    import bench
+    from windtunnel._lib import WT_FLAG_FORCES, WT_FLAG_MASS
+    n = dims[0] * dims[1] * dims[2]
+    assert bench.cache_policy(name, n) == 0
+    mask = _car_mask(dims, seed=11)
+    uin, tau, smag = 0.05, 0.52, 0.16
+    t = _tunnel(dims, name, mask, uin)
+    t.step(3, uin, tau, smag, flags=WT_FLAG_FORCES | WT_FLAG_MASS)
+    f = t.raw_populations().cpu().numpy()
+    mac = t.mac.cpu().numpy()
+    sums = t.sums.cpu().numpy()
+    del t
+    torch.cuda.empty_cache()
+    O.set_threads(16)
+    o = _oracle(dims, form, mask, uin)
+    o.step(3, uin, tau, smag)
+    assert _raw_equal(f, o.raw()), dims
+    assert _raw_equal(mac, o.mac.astype(np.float32)), dims
+    np.testing.assert_allclose(sums[:3], o.forces(), rtol=1e-12, atol=1e-15)
+    np.testing.assert_allclose(sums[3], o.mass, rtol=1e-12)
+
+
 @pytest.mark.parametrize('form', ['fp16', 'bf16'])
 def test_global_addressing_matches_buffer_addressing(gpu, form):
     from windtunnel._lib import WT_FLAG_FORCES

@@ -14,8 +14,8 @@ server/app.py:180 counts them); the same workload with FP16-deviation storage
 is reported beside it. roofline: algorithmic bytes of the step kernel
 (169 B/cell FP32, 93 B/cell FP16: 19 populations read and written, the class
 byte, the 16 B macroscopic store) over its mean launch time, from HIP events on
-the stream the kernel runs on, bracketing every 10th launch of the timed
-region (steps 5, 15, 25, ...: force-free; a pair of events on every step slows
+the stream the kernel runs on, bracketing every 5th launch of the timed
+region (session steps 0 mod 5: force-free; a pair of events on every step slows
 the loop by ~0.7 %). Each step's particles are advected by their own
 launch (k_advect) right after the step, inside the timed region; a session
 built with fuse=True instead advects them inside the next step's launch
@@ -37,17 +37,10 @@ sys.path.insert(0, REPO)
 PEAK_HBM_GBS = 8000.0
 BYTES_PER_CELL = {'fp32': 169, 'fp16': 93, 'bf16': 93, 'fp64': 321}
 BYTES_PER_PARTICLE = 40
-EVENT_EVERY, EVENT_PHASE = 10, 5     # HIP events around steps numbered 5 mod 10
+EVENT_EVERY, EVENT_PHASE = 5, 0      # HIP events around steps numbered 0 mod 5 (force-free)
 FORM_IDS = {'fp32': 0, 'fp16': 1, 'bf16': 2, 'fp64': 3}
-ELEM_BYTES = {'fp32': 4, 'fp16': 2, 'bf16': 2, 'fp64': 8}
 
 
-def cache_policy(form, n):
-    """The cache policy the library's single-tunnel step runs on this tunnel (the
-    third template argument of k_step), by the bytes of both population buffers:
-    1 streaming above 2 GiB, 2 all cached within 224 MiB, else 0."""
-    b = 2 * 19 * n * ELEM_BYTES[form]
-    return 1 if b >= 1 << 31 else (2 if b <= 224 << 20 else 0)
 DIMS = (512, 256, 256)
 UIN, TAU, SMAG = 0.05, 0.51, 0.16
 BODY_SEED = 2025
@@ -74,12 +67,14 @@ def run_gpu(form, steps, warmup, mask, counts, lref):
     s = Session(DIMS, mask, form, UIN, TAU, SMAG, particles=PARTICLES, slices=slices, frame_every=10,
                 lref=lref, seed=7, frontal=counts['frontal'])
     stream = torch.cuda.current_stream()
+    torch.cuda.synchronize()
     for _ in range(warmup):
         s.step()
     s.flush()
     torch.cuda.synchronize()
-    # steps whose session step number is 5 mod 10 (frames are 9 mod 10), never
-    # the first timed launch (a fused session's is a plain step: nothing pending)
+    # steps whose session step number is 0 mod 5 (frames, the only force steps,
+    # are 9 mod 10), never the first timed launch (a fused session's is a plain
+    # step: nothing pending)
     c0 = s.step_count
     sampled = [k for k in range(1, steps) if (c0 + k) % EVENT_EVERY == EVENT_PHASE and not s.is_frame(c0 + k)]
     if not sampled:                        # runs of a few steps: the last one
@@ -99,6 +94,7 @@ def run_gpu(form, steps, warmup, mask, counts, lref):
     gpu_ms = t_start.elapsed_time(t_end)
     plain = [ev[k][0].elapsed_time(ev[k][1]) for k in sampled]
     step_ms = sum(plain) / max(1, len(plain))
+    srt = sorted(plain)
     cd, cl = s.coefficients()
     viewer = s.viewer_coefficients()
     n = s.n
@@ -106,11 +102,12 @@ def run_gpu(form, steps, warmup, mask, counts, lref):
     res = {
         'form': form, 'wall_s': wall, 'gpu_ms': gpu_ms, 'mlups': n * steps / wall / 1e6,
         'ms_per_step': wall * 1e3 / steps, 'step_kernel_ms': step_ms,
-        'kernel_launches_timed': len(plain),
+        'kernel_launches_timed': len(plain), 'kernel_ms_min': srt[0], 'kernel_ms_median': srt[len(srt) // 2],
+        'kernel_ms_max': srt[-1],
         'kernel_mlups': n / (step_ms * 1e-3) / 1e6, 'fused_advect': s.fuse, 'bytes_per_launch': nbytes,
-        'kernel': (f"k_step_adv<{FORM_IDS[form]}, 1, {cache_policy(form, n)}> "
+        'kernel': (f"k_step_adv<{FORM_IDS[form]}, {s.T.shape[0]}, {s.T.cache_policy}> "
                    "(step + previous step's advection)") if s.fuse
-                  else f'k_step<{FORM_IDS[form]}, 1, {cache_policy(form, n)}> (fused stream/collide)',
+                  else f'k_step<{FORM_IDS[form]}, {s.T.shape[0]}, {s.T.cache_policy}> (fused stream/collide)',
         'achieved_gbs': nbytes / (step_ms * 1e-3) / 1e9,
         'cd': float(cd), 'cl': float(cl), 'viewer_cd_cl': None if viewer is None else [float(v) for v in viewer],
     }

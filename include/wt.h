@@ -47,10 +47,21 @@ extern "C" {
  *             next to the domain faces; face cells read the previous step's
  *             values from it.
  * partials:   per-tile FP64 reduction slots (force x,y,z and mass).
- * sums:       4 doubles on the device: fx, fy, fz, mass. */
+ * sums:       4 doubles on the device: fx, fy, fz, mass.
+ * cpt, tile_order: launch shape of this tunnel's step kernels, chosen per
+ *             size class by the autotuner; no result depends on it.
+ *             cpt = cells per thread (1, 2 or 4 consecutive 256-cell slots per
+ *             workgroup; 0 means 1). tile_order bits: bit 0 XCD-contiguous
+ *             tiles, bit 1 sweep direction alternating with the buffer parity,
+ *             bit 2 64-bit global population addresses even where 32-bit
+ *             buffer offsets would do (the form large tunnels take by
+ *             themselves). Every call reads the shape from the grid it is
+ *             given: the library keeps no launch state between calls. */
 typedef struct wt_grid {
   int32_t nx, ny, nz;
   int32_t form;
+  int32_t cpt;
+  int32_t tile_order;
   void* f[2];
   uint8_t* cls;
   float* mac;
@@ -87,6 +98,13 @@ int wt_buffer_sizes(int nx, int ny, int nz, int form, int64_t sizes[6]);
 int64_t wt_state_bytes(int nx, int ny, int nz, int form);
 /* Number of reduction tiles of a tunnel (partials holds 4 doubles each). */
 int64_t wt_tile_count(int nx, int ny, int nz);
+/* Cache policy wt_step uses for a tunnel of this size and storage form, by
+ * the bytes of its two population buffers: 1 (above 2 GiB) streams the
+ * populations' c_x = 0 planes and the macroscopic field past the caches,
+ * 2 (within 224 MiB) keeps everything cached, 0 otherwise (population stores
+ * non-temporal). Batched steps always use 0. No counterpart in the reference;
+ * it names the kernel variant a measurement refers to. */
+int wt_cache_policy(int nx, int ny, int nz, int form);
 
 /* ---- state ---------------------------------------------------------------- */
 /* Equilibrium at rho = 1, u = (uin, 0, 0) in both population buffers, mac and
@@ -121,16 +139,23 @@ int wt_forces(const wt_grid* g, int cur, void* stream);
 int wt_read_sums(const wt_grid* g, double out[4], void* stream);
 
 /* ---- batched tunnels ------------------------------------------------------ */
-/* Many resident tunnels advance one step each in ONE launch. Replaces running
- * one Solver.step per tunnel (lbm.py:442-447) for the serving loop
- * (app.py:150-202 generalised to many sessions). */
+/* Many resident tunnels advance one step each in ONE persistent launch: a
+ * grid of `blocks` workgroups (about one full occupancy of the GPU) strides
+ * over the work items, tile k of tunnel t, where tunnel t's tiles hold
+ * 256 * grids[t].cpt cells (each tunnel keeps the launch shape of its size
+ * class). Replaces running one Solver.step per tunnel (lbm.py:442-447) for
+ * the serving loop (app.py:150-202 generalised to many sessions).
+ * total_tiles: work items of one step. blocks: workgroups of the launch, set
+ * by wt_batch_prepare from the device's occupancy (a caller may lower or
+ * raise it before wt_batch_step; results do not depend on it). tile_order:
+ * bit 1 sweeps the items backwards on odd parities (set by prepare). */
 typedef struct wt_batch_info {
   int64_t total_tiles;
   int32_t max_chunks;
   int32_t form;
   int32_t ntunnels;
-  int32_t cpt;
-  int32_t remap;
+  int32_t blocks;
+  int32_t tile_order;
   int32_t reserved;
 } wt_batch_info;
 /* Device workspace bytes for a batch of ntunnels (descriptor tables). */
@@ -146,15 +171,6 @@ int wt_batch_prepare(const wt_grid* grids, const wt_params* params, const int* c
  * reads buffer cur[k]^p. flags as for wt_step; sums land in each grid's sums.
  * Kernel launches only: a caller may capture it in a graph. */
 int wt_batch_step(void* workspace, const wt_batch_info* info, int parity, int flags, void* stream);
-
-/* Launch shape of the step kernels of one storage form: cells per thread
- * (1, 2 or 4) and flags: bit 0 XCD-contiguous tiles, bit 1 sweep direction
- * alternating with the buffer parity, bit 2 (wt_step only) 64-bit global
- * population addresses even where 32-bit buffer offsets would do (the form
- * large tunnels take). Chosen by the autotuner; results do not depend on it.
- * Applies to later wt_step / wt_batch_prepare. */
-int wt_set_launch_shape(int form, int cpt, int remap);
-int wt_get_launch_shape(int form, int* cpt, int* remap);
 
 /* ---- particles, slices, surface pressure ---------------------------------- */
 /* RK2 particle advection with respawn (advect_kernel, lbm.py:233-263).

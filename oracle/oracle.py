@@ -58,6 +58,7 @@ def lib():
         L.oracle_f2bf.restype = ctypes.c_uint16
         L.oracle_bf2f.argtypes = [ctypes.c_uint16]
         L.oracle_bf2f.restype = f32
+        L.oracle_clamp_hits.argtypes = [ctypes.POINTER(ctypes.c_int64)]
         L.oracle_set_threads.argtypes = [i32]
         L.oracle_get_threads.restype = i32
         _lib = L
@@ -66,6 +67,14 @@ def lib():
 
 def _p(a):
     return a.ctypes.data_as(vp)
+
+
+def clamp_hits():
+    """(velocity clamps, face density clamps) counted by every oracle step since the
+    last call (lbm.py:130-135, lbm.py:88)."""
+    out = (ctypes.c_int64 * 2)()
+    lib().oracle_clamp_hits(out)
+    return int(out[0]), int(out[1])
 
 
 def set_threads(n):

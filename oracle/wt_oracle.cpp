@@ -112,27 +112,144 @@ float bf2f(uint16_t b) { return bitsf((uint32_t)b << 16); }
 
 // ---- storage forms ----------------------------------------------------------
 // FORM 0: FP32 absolute f_i; 1: FP16 deviation; 2: BF16 deviation; 3: FP64 absolute.
+// load/store: f_i of a stored element. pop/put: the value the collision works on
+// (f_i for the absolute forms; the deviation g_i = f_i - w_i for FP16/BF16, whose
+// collision is carried out on deviations: collide_dev below). excess: f_i - w_i,
+// the momentum-exchange term of lbm.py:189 (exactly g_i in the deviation forms).
 template <int FORM> struct Form;
 template <> struct Form<0> {
   typedef float elem; typedef float real;
+  static constexpr bool dev = false;
   static float load(elem e, int) { return e; }
   static elem store(float f, int) { return f; }
+  static float pop(elem e) { return e; }
+  static elem put(float v) { return v; }
+  static float excess(elem e, int i) { return e - Lit<float>::w(i); }
 };
 template <> struct Form<1> {
   typedef uint16_t elem; typedef float real;
+  static constexpr bool dev = true;
   static float load(elem e, int i) { return Lit<float>::w(i) + h2f(e); }
   static elem store(float f, int i) { return f2h(f - Lit<float>::w(i)); }
+  static float pop(elem e) { return h2f(e); }
+  static elem put(float v) { return f2h(v); }
+  static float excess(elem e, int) { return h2f(e); }
 };
 template <> struct Form<2> {
   typedef uint16_t elem; typedef float real;
+  static constexpr bool dev = true;
   static float load(elem e, int i) { return Lit<float>::w(i) + bf2f(e); }
   static elem store(float f, int i) { return f2bf(f - Lit<float>::w(i)); }
+  static float pop(elem e) { return bf2f(e); }
+  static elem put(float v) { return f2bf(v); }
+  static float excess(elem e, int) { return bf2f(e); }
 };
 template <> struct Form<3> {
   typedef double elem; typedef double real;
+  static constexpr bool dev = false;
   static double load(elem e, int) { return e; }
   static elem store(double f, int) { return f; }
+  static double pop(elem e) { return e; }
+  static elem put(double v) { return v; }
+  static double excess(elem e, int i) { return e - Lit<double>::w(i); }
 };
+
+// Index-ordered signed sum of v over the directions with nonzero k (a lattice
+// moment), starting from its first term instead of 0.0: the deviation form's sums.
+template <typename R, typename K> R moment(const R* v, K k) {
+  R acc = 0;
+  bool any = false;
+  for (int i = 0; i < 19; i++) {
+    const int c = k(i);
+    if (c == 0) continue;
+    if (!any) { acc = c > 0 ? v[i] : -v[i]; any = true; }
+    else acc = c > 0 ? acc + v[i] : acc - v[i];
+  }
+  return acc;
+}
+
+// Collision of the FP16/BF16 deviation forms (this build's own storage forms; the
+// reference stores FP32 only). The regularized Smagorinsky collision of
+// lbm.py:125-165 restated on g_i = f_i - w_i, all arithmetic FP32:
+//   dr = sum g, rho = 1 + dr, u = (sum c g) / rho with the clamp of lbm.py:130-135,
+//   eps_i = e_i - w_i = w_i dr + (w_i rho) (cu_i + (h_i - usq)), cu_i = 3 c_i.u
+//   (c_i.u summed from its first component), h_i = (0.5 cu_i) cu_i,
+//   d_i = g_i - eps_i, Pi = sum c c d, Smagorinsky tau_e and sponge as lbm.py:147-154,
+//   g_out_i = eps_i + (4.5 w_i keep) (c_i c_i : Pi - tr3)   (i = 0: eps_0 - (4.5 w_0 keep) tr3).
+// Since sum w_i = 1, sum c_i w_i = 0 and w_i = w_opp(i), this is the reference's
+// collision in exact arithmetic; the device kernel evaluates the same sequence.
+bool collide_dev(const float* g, int x, int nx, float tau, float smag, float* out, float* mo) {
+  float dr = g[0];
+  for (int i = 1; i < 19; i++) dr = dr + g[i];
+  const float rho = 1.0f + dr;
+  const float ir = 1.0f / rho;
+  float ux = moment(g, [](int i) { return CX[i]; }) * ir;
+  float uy = moment(g, [](int i) { return CY[i]; }) * ir;
+  float uz = moment(g, [](int i) { return CZ[i]; }) * ir;
+  const float um = std::sqrt(ux * ux + uy * uy + uz * uz);
+  const bool clamped = um > 0.35f;
+  if (clamped) {
+    const float sc = 0.35f / um;
+    ux = ux * sc; uy = uy * sc; uz = uz * sc;
+  }
+  const float usq = 1.5f * (ux * ux + uy * uy + uz * uz);
+  float eps[19], d[19];
+  for (int i = 0; i < 19; i++) {
+    const float w = Lit<float>::w(i);
+    const float a = w * dr, b = w * rho;
+    if (i == 0) {
+      eps[i] = a - b * usq;
+    } else {
+      float sdot = 0.0f;
+      bool any = false;
+      const float comp[3] = {ux, uy, uz};
+      const int cc[3] = {CX[i], CY[i], CZ[i]};
+      for (int k = 0; k < 3; k++) {
+        if (!cc[k]) continue;
+        if (!any) { sdot = cc[k] > 0 ? comp[k] : -comp[k]; any = true; }
+        else sdot = cc[k] > 0 ? sdot + comp[k] : sdot - comp[k];
+      }
+      const float cu = 3.0f * sdot;
+      const float h = 0.5f * cu * cu;
+      eps[i] = a + b * (cu + (h - usq));
+    }
+    d[i] = g[i] - eps[i];
+  }
+  const float pxx = moment(d, [](int i) { return CX[i] * CX[i]; });
+  const float pyy = moment(d, [](int i) { return CY[i] * CY[i]; });
+  const float pzz = moment(d, [](int i) { return CZ[i] * CZ[i]; });
+  const float pxy = moment(d, [](int i) { return CX[i] * CY[i]; });
+  const float pxz = moment(d, [](int i) { return CX[i] * CZ[i]; });
+  const float pyz = moment(d, [](int i) { return CY[i] * CZ[i]; });
+  const float qn = std::sqrt(pxx * pxx + pyy * pyy + pzz * pzz + 2.0f * (pxy * pxy + pxz * pxz + pyz * pyz));
+  float tau0 = tau;
+  const float sp = (float)x - (float)nx + 24.0f;
+  if (sp > 0.0f) tau0 = tau0 + sp * 0.015f;
+  const float tau_e = 0.5f * (tau0 + std::sqrt(tau0 * tau0 + 25.4558441f * smag * smag * qn * ir));
+  const float keep = 1.0f - 1.0f / tau_e;
+  const float tr3 = (pxx + pyy + pzz) * (1.0f / 3.0f);
+  for (int i = 0; i < 19; i++) {
+    const float k = (float)(4.5 * w_d(i)) * keep;
+    if (i == 0) { out[0] = eps[0] - k * tr3; continue; }
+    bool any = false;
+    float acc = 0.0f;
+    if (CX[i]) { acc = pxx; any = true; }
+    if (CY[i]) { acc = any ? acc + pyy : pyy; any = true; }
+    if (CZ[i]) { acc = any ? acc + pzz : pzz; any = true; }
+    if (CX[i] && CY[i]) { float t = (float)(2 * CX[i] * CY[i]) * pxy; acc = any ? acc + t : t; any = true; }
+    if (CX[i] && CZ[i]) { float t = (float)(2 * CX[i] * CZ[i]) * pxz; acc = any ? acc + t : t; any = true; }
+    if (CY[i] && CZ[i]) { float t = (float)(2 * CY[i] * CZ[i]) * pyz; acc = any ? acc + t : t; any = true; }
+    out[i] = eps[i] + k * (acc - tr3);
+  }
+  mo[0] = ux; mo[1] = uy; mo[2] = uz; mo[3] = rho;
+  return clamped;
+}
+
+// How often the numerical guards fired in the steps since the last
+// oracle_clamp_hits() call: [0] velocity clamp |u| > 0.35 (lbm.py:130-135),
+// [1] face density clamp of a neighbour rho outside [0.9, 1.1] on a face whose
+// density is not overridden (lbm.py:88; the inlet and outlet set rho = 1).
+int64_t g_hits[2] = {0, 0};
 
 template <typename T> T clampT(T v, T a, T b) { return std::min(std::max(v, a), b); }
 
@@ -170,9 +287,11 @@ void step_t(const void* finv, void* foutv, const uint8_t* solid, const void* mac
   const R uin = (R)uin_f, tau = (R)tau_f, smag = (R)smag_f;
   const R zero = 0, one = 1;
   std::vector<double> zmass(nz, 0.0);
+  std::vector<int64_t> zvel(nz, 0), zden(nz, 0);
 #pragma omp parallel for schedule(static)
   for (int z = 0; z < nz; z++) {
     double mass = 0.0;
+    int64_t vel = 0, den = 0;
     for (int y = 0; y < ny; y++)
       for (int x = 0; x < nx; x++) {
         const int64_t idx = x + (int64_t)nx * (y + (int64_t)ny * z);
@@ -186,6 +305,7 @@ void step_t(const void* finv, void* foutv, const uint8_t* solid, const void* mac
           int cx = clampT(x, 1, nx - 2), cy = clampT(y, 1, ny - 2), cz = clampT(z, 1, nz - 2);
           const R* m = mac_prev + 4 * (cx + (int64_t)nx * (cy + (int64_t)ny * cz));
           R br = clampT(m[3], (R)0.9, (R)1.1);
+          if (x != 0 && x != nx - 1 && br != m[3]) den++;
           R bux = m[0], buy = m[1], buz = m[2];
           if (y == 0 || y == ny - 1) buy = 0;
           if (z == 0 || z == nz - 1) buz = 0;
@@ -207,16 +327,24 @@ void step_t(const void* finv, void* foutv, const uint8_t* solid, const void* mac
         }
         // interior fluid: pull streaming with bounce-back and belt, lbm.py:110-124
         R f[19];
-        f[0] = S::load(fin[idx], 0);
+        f[0] = S::pop(fin[idx]);
         for (int i = 1; i < 19; i++) {
           int64_t s = (x - CX[i]) + (int64_t)nx * ((y - CY[i]) + (int64_t)ny * (z - CZ[i]));
           int t = solid[s];
           if (t == 0) {
-            f[i] = S::load(fin[i * n + s], i);
+            f[i] = S::pop(fin[i * n + s]);
           } else {
-            f[i] = S::load(fin[OPP[i] * n + idx], OPP[i]);
+            // bounce-back: w_i = w_opp(i), so a deviation bounces back unchanged
+            f[i] = S::pop(fin[OPP[i] * n + idx]);
             if (CX[i] && t == 2) f[i] = f[i] + (R)(6.0 * w_d(i) * CX[i]) * uin;
           }
+        }
+        if constexpr (S::dev) {
+          float out[19];
+          if (collide_dev(f, x, nx, (float)tau, (float)smag, out, mo)) vel++;
+          for (int i = 0; i < 19; i++) fout[i * n + idx] = S::put(out[i]);
+          mass += (double)mo[3];
+          continue;
         }
         // moments, lbm.py:125-136 (signed sums in index order, as lin() emits them)
         R rho = zero;
@@ -235,6 +363,7 @@ void step_t(const void* finv, void* foutv, const uint8_t* solid, const void* mac
         R ux = sx * ir, uy = sy * ir, uz = sz * ir;
         R um = std::sqrt(ux * ux + uy * uy + uz * uz);
         if (um > (R)0.35) {
+          vel++;
           R sc = (R)0.35 / um;
           ux = ux * sc; uy = uy * sc; uz = uz * sc;
         }
@@ -288,13 +417,16 @@ void step_t(const void* finv, void* foutv, const uint8_t* solid, const void* mac
           if (CY[i] && CZ[i]) { R t = (R)(2 * CY[i] * CZ[i]) * pyz; acc = any ? acc + t : t; any = true; }
           qp = (any ? acc : zero) - tr3;
           R v = e[i] + (R)(4.5 * w_d(i)) * keep * qp;
-          fout[i * n + idx] = S::store(v, i);
+          fout[i * n + idx] = S::put(v);
         }
         mo[0] = ux; mo[1] = uy; mo[2] = uz; mo[3] = rho;
         mass += (double)rho;
       }
     zmass[z] = mass;
+    zvel[z] = vel;
+    zden[z] = den;
   }
+  for (int z = 0; z < nz; z++) { g_hits[0] += zvel[z]; g_hits[1] += zden[z]; }
   if (mass_out) {
     double m = 0.0;
     for (int z = 0; z < nz; z++) m += zmass[z];
@@ -317,7 +449,7 @@ void force_t(const void* fv, const uint8_t* solid, int nx, int ny, int nz, doubl
         for (int i = 1; i < 19; i++) {
           int64_t s = (x + CX[i]) + (int64_t)nx * ((y + CY[i]) + (int64_t)ny * (z + CZ[i]));
           if (solid[s] != 1) continue;
-          R v = (R)2.0 * (S::load(f[i * n + idx], i) - Lit<R>::w(i));
+          R v = (R)2.0 * S::excess(f[i * n + idx], i);
           if (CX[i]) fx += CX[i] > 0 ? (double)v : -(double)v;
           if (CY[i]) fy += CY[i] > 0 ? (double)v : -(double)v;
           if (CZ[i]) fz += CZ[i] > 0 ? (double)v : -(double)v;
@@ -559,6 +691,10 @@ void oracle_cp(const float* mac, const uint8_t* solid, const float* verts, const
 }
 
 // Conversions exposed for tests of the storage forms.
+void oracle_clamp_hits(int64_t* out) {
+  out[0] = g_hits[0]; out[1] = g_hits[1];
+  g_hits[0] = 0; g_hits[1] = 0;
+}
 uint16_t oracle_f2h(float f) { return f2h(f); }
 float oracle_h2f(uint16_t h) { return h2f(h); }
 uint16_t oracle_f2bf(float f) { return f2bf(f); }

@@ -122,3 +122,34 @@ def test_batch_with_a_large_tunnel_equals_alone(gpu):
         bs = b.last_sums().cpu().numpy()
         for j, t in enumerate(alone):
             assert _same([t.sums.cpu().numpy()], [bs[j]]), (k, t.nx)
+
+
+@pytest.mark.parametrize('blocks', [1, 37, 0])
+def test_persistent_batch_any_grid_and_mixed_shapes(gpu, blocks):
+    """One persistent launch per step with each tunnel's own cells per thread (and
+    64-bit addressing for one of them): any number of workgroups -- one, a few that
+    each loop over many tiles, or the occupancy default (0 here) -- and both sweep
+    directions give every tunnel the bits it gets alone."""
+    from windtunnel._lib import WT_FLAG_FORCES, WT_FLAG_MASS
+    from windtunnel.batch import Batch
+    shapes = [(1, 0), (2, 0), (4, 2), (2, 4), (1, 2)]
+    alone = _make('fp32', pre_steps=(1,))
+    mixed = _make('fp32', pre_steps=(1,))
+    for a, m, sh in zip(alone, mixed, shapes):
+        a.shape = (1, 0)
+        m.shape = sh
+    b = Batch(mixed, PARAMS)
+    if blocks:
+        b.info.blocks = blocks
+    for order in (2, 0):
+        b.info.tile_order = order
+        flags = WT_FLAG_FORCES | WT_FLAG_MASS
+        for k in range(3):
+            b.step(flags)
+            for t, p in zip(alone, PARAMS):
+                t.step(1, *p, flags=flags)
+            torch.cuda.synchronize()
+            bs = b.last_sums().cpu().numpy()
+            for j, (ta, tm) in enumerate(zip(alone, mixed)):
+                assert _same(_state(ta)[:2], _state(tm)[:2]), (blocks, order, k, j)
+                assert _same([ta.sums.cpu().numpy()], [bs[j]]), (blocks, order, k, j)

@@ -46,8 +46,44 @@ def test_error_reporting_without_gpu_calls():
     assert 'at least 3' in _lib.last_error()
     assert lib.wt_buffer_sizes(8, 8, 8, 9, s) < 0
     assert 'storage form' in _lib.last_error()
-    assert lib.wt_set_launch_shape(0, 3, 0) < 0
     assert lib.wt_version() == 1
+
+
+def test_launch_shape_is_checked_per_grid_before_any_launch():
+    """The launch shape travels in the grid descriptor of every call; a bad one is
+    rejected before anything reaches the GPU (the fake buffer addresses are never
+    dereferenced)."""
+    from windtunnel import _lib
+    lib = _lib.load()
+    fake = ctypes.c_void_p(0x1000)
+    for cpt, order, what in ((3, 0, 'cells per thread'), (1, 8, 'tile order')):
+        g = _lib.Grid(16, 8, 8, 0, cpt, order, (ctypes.c_void_p * 2)(0x1000, 0x2000), 0x3000, 0x4000,
+                      (ctypes.c_void_p * 2)(0x5000, 0x6000), 0x7000, 0x8000)
+        p = _lib.Params(0.05, 0.52, 0.16)
+        assert lib.wt_step(ctypes.byref(g), 0, 1, ctypes.byref(p), 0, None) < 0
+        assert what in _lib.last_error()
+    del fake
+
+
+@pytest.mark.parametrize('dims,form,policy', [
+    ((128, 64, 64), 0, 2),      # 2 x 19 x 4 B x 0.52M = 80 MB: all cached
+    ((160, 96, 96), 0, 2),      # 213.75 MiB, within the 224 MiB bound
+    ((200, 96, 96), 0, 0),      # 280 MB
+    ((256, 112, 136), 1, 0),    # FP16, 296 MB
+    ((256, 128, 128), 1, 0),    # FP16, 319 MB
+    ((384, 192, 192), 0, 1),    # 2.15 GB >= 2 GiB: streaming
+    ((512, 256, 256), 1, 1),    # FP16 33.5M cells: 2.55 GB, above 2 GiB
+    ((512, 256, 256), 0, 1),
+])
+def test_cache_policy_boundaries(dims, form, policy):
+    """wt_cache_policy: 2 within 224 MiB of populations (both buffers), 1 from
+    2 GiB, else 0; computed from the element size of the storage form."""
+    from windtunnel import _lib
+    lib = _lib.load()
+    n = dims[0] * dims[1] * dims[2]
+    b = 2 * 19 * n * {0: 4, 1: 2, 2: 2, 3: 8}[form]
+    expect = 1 if b >= 1 << 31 else (2 if b <= 224 << 20 else 0)
+    assert lib.wt_cache_policy(*dims, form) == expect == policy
 
 
 @pytest.mark.parametrize('form,elem', [(0, 4), (1, 2), (2, 2), (3, 8)])

@@ -183,14 +183,82 @@ def test_outlet_sponge_raises_tau():
 
 
 def test_velocity_clamp():
-    """|u| > 0.35 is scaled back to 0.35 (lbm.py:130-135)."""
+    """|u| > 0.35 is scaled back to 0.35 (lbm.py:130-135), and the oracle counts
+    every clamped cell."""
     dims = (12, 7, 7)
     f = np.tile(equilibrium(1.0, [0.3, 0.3, 0.0])[:, None], (1, int(np.prod(dims))))
+    O.clamp_hits()
     t = _run_one(dims, f, uin=0.05)
     inner = _interior(dims)
     u = t.mac[inner, :3]
     assert np.allclose(np.linalg.norm(u, axis=1), 0.35, atol=1e-14)
     assert np.allclose(u[:, 0], u[:, 1], atol=1e-15)
+    assert O.clamp_hits() == (int(inner.sum()), 0)
+
+
+def test_face_density_clamp_counts():
+    """A face cell whose interior neighbour had rho outside [0.9, 1.1] in the previous
+    step takes the clamped density (lbm.py:88); inlet and outlet faces set rho = 1
+    and do not count."""
+    dims = (14, 8, 9)
+    nx, ny, nz = dims
+    t = O.OracleTunnel(dims, O.FP64)
+    t.init(0.05)
+    # previous-step field: rho 1.3 on the layer y = ny - 2 and 0.8 on z = 1
+    m = t.mac.reshape(nz, ny, nx, 4)
+    m[:, ny - 2, :, 3] = 1.3
+    m[1, :, :, 3] = 0.8
+    O.clamp_hits()
+    t.step(1, 0.05, 0.52, 0.17)
+    vel, den = O.clamp_hits()
+    mo = t.mac.reshape(nz, ny, nx, 4)
+    # y = ny - 1 faces (x inside 1..nx-2) read 1.3 -> 1.1; z = 0 faces read the z = 1 layer -> 0.9
+    top = mo[2:nz - 1, ny - 1, 1:nx - 1, 3]
+    assert np.all(top == 1.1)
+    bottom = mo[0, 1:ny - 2, 1:nx - 1, 3]
+    assert np.all(bottom == 0.9)
+    # the count: every face cell off the inlet and outlet whose clamped neighbour lies
+    # on either modified layer
+    z_, y_, x_ = np.meshgrid(np.arange(nz), np.arange(ny), np.arange(nx), indexing='ij')
+    face = (x_ == 0) | (x_ == nx - 1) | (y_ == 0) | (y_ == ny - 1) | (z_ == 0) | (z_ == nz - 1)
+    cy, cz = np.clip(y_, 1, ny - 2), np.clip(z_, 1, nz - 2)
+    prev_rho = np.where(cy == ny - 2, 1.3, np.where(cz == 1, 0.8, 1.0))
+    # where the two layers cross the field holds 0.8 (written last); both values clamp
+    expect = face & (x_ > 0) & (x_ < nx - 1) & ((prev_rho < 0.9) | (prev_rho > 1.1))
+    assert vel == 0 and den == int(expect.sum()) > 0
+
+
+def test_oracle_mass_is_the_sum_of_the_stored_populations():
+    """The mass the oracle step reports (FP64, faces included, solids excluded) equals
+    sum over non-solid cells of sum_i f_i of what the step stored: collision conserves
+    mass and a face cell's equilibrium sums to its boundary density. Counting solid
+    cells (2,109 of them at rho ~ 1) or dropping faces would be off by > 1e-3."""
+    from windtunnel import geometry
+    dims = (48, 20, 22)
+    mask, _ = geometry.sphere(dims, (14, 10, 11), 8)
+    t = O.OracleTunnel(dims, O.FP64)
+    t.init(0.05)
+    t.set_solid(mask)
+    t.step(7, 0.05, 0.52, 0.17)
+    fluid = mask == 0
+    total = t.populations()[:, fluid].sum()
+    assert abs(t.mass - total) <= 1e-12 * total
+    assert abs(t.mass - t.mac[fluid, 3].sum()) <= 1e-12 * total
+
+
+def test_oracle_mass_at_rest_is_the_fluid_cell_count():
+    """At the rest fixed point (uin = 0) every non-solid cell has rho = 1."""
+    from windtunnel import geometry
+    dims = (30, 12, 14)
+    mask, counts = geometry.sphere(dims, (10, 6, 7), 6)
+    for form, tol in ((O.FP64, 1e-13), (O.FP32, 1e-6)):
+        t = O.OracleTunnel(dims, form)
+        t.init(0.0)
+        t.set_solid(mask)
+        t.step(3, 0.0, 0.52, 0.17)
+        nfluid = int((mask == 0).sum())
+        assert nfluid == int(np.prod(dims)) - counts['count']
+        assert abs(t.mass - nfluid) <= tol * nfluid
 
 
 # ---- boundaries ---------------------------------------------------------------------------
@@ -282,6 +350,80 @@ def test_fp32_oracle_tracks_fp64():
         r.append((t.mac.astype(np.float64), t.forces()))
     assert np.abs(r[0][0] - r[1][0]).max() < 2e-6
     assert np.allclose(r[0][1], r[1][1], rtol=1e-4, atol=5e-6)
+
+
+def _bf16_round(x):
+    """float64 -> bfloat16 bits, round to nearest even (numpy has no bfloat16)."""
+    b = np.asarray(x, np.float32).view(np.uint32).astype(np.uint64)
+    return ((b + 0x7fff + ((b >> 16) & 1)) >> 16).astype(np.uint16)
+
+
+@pytest.mark.parametrize('form', [O.FP16, O.BF16])
+def test_deviation_collision_equals_reference_collision(form):
+    """The FP16/BF16 forms collide on deviations g = f - w (FP32 arithmetic, no weight
+    added back): one step from a state exactly representable in the storage form
+    must equal the FP64 form's reference-order step (lbm.py:110-166) on f = w + g,
+    to within half a storage ulp of each output plus FP32 rounding. A dropped term,
+    a sign or a wrong weight class in the deviation algebra is 1e-4 or more."""
+    from windtunnel import geometry
+    dims = (40, 11, 12)
+    rng = np.random.default_rng(21)
+    n = int(np.prod(dims))
+    u = np.array([0.06, -0.02, 0.015])
+    f = equilibrium(1.01, u)[:, None] * (1 + 0.01 * rng.standard_normal((19, n)))
+    mask, _ = geometry.sphere(dims, (14, 5, 6), 5)
+    mask = mask.copy()
+    mask.reshape(dims[2], dims[1], dims[0])[:, 0, :] = 2        # a moving belt under it
+    if form == O.FP16:
+        graw = (f - W[:, None]).astype(np.float16)
+        g = graw.astype(np.float64)
+        bits = graw.view(np.uint16)
+    else:
+        bits = _bf16_round(f - W[:, None])
+        g = (bits.astype(np.uint32) << 16).view(np.float32).astype(np.float64)
+    fexact = W[:, None] + g
+    lo = O.OracleTunnel(dims, form)
+    lo.init(0.05)
+    lo.set_solid(mask)
+    lo.f[0][:] = bits.reshape(-1)
+    lo.f[1][:] = bits.reshape(-1)
+    lo.step(1, 0.05, 0.52, 0.17)
+    hi = O.OracleTunnel(dims, O.FP64)
+    hi.init(0.05)
+    hi.set_solid(mask)
+    hi.f[0][:] = fexact.reshape(-1)
+    hi.f[1][:] = fexact.reshape(-1)
+    hi.step(1, 0.05, 0.52, 0.17)
+    inner = _interior(dims) & (mask == 0)
+    got = lo.populations()[:, inner] - W[:, None]          # stored deviations, exactly
+    want = hi.populations()[:, inner] - W[:, None]
+    if form == O.FP16:
+        ulp = np.spacing(np.abs(want).astype(np.float16)).astype(np.float64)
+    else:
+        ulp = np.abs(want) * 2.0 ** -7
+    err = np.abs(got - want)
+    assert (err <= 0.5 * ulp + 3e-7).all(), err.max()
+    assert np.abs(lo.mac[inner, 3] - hi.mac[inner, 3]).max() < 3e-7         # rho = 1 + sum g
+    assert np.abs(lo.mac[inner, :3] - hi.mac[inner, :3]).max() < 3e-8
+    # the force is 2 g on each link: exactly twice the stored deviations
+    assert np.allclose(lo.forces(), hi.forces(), rtol=2e-3, atol=1e-4)
+
+
+@pytest.mark.parametrize('form,tol', [(O.FP16, 2e-4), (O.BF16, 2e-3)])
+def test_deviation_forms_track_fp64(form, tol):
+    """Over 50 steps of flow past a sphere the FP16 (BF16) deviation forms stay
+    within their storage precision of the FP64 form."""
+    from windtunnel import geometry
+    dims = (48, 20, 20)
+    mask, _ = geometry.sphere(dims, (14, 10, 10), 8)
+    r = []
+    for f in (form, O.FP64):
+        t = O.OracleTunnel(dims, f)
+        t.init(0.05)
+        t.set_solid(mask)
+        t.step(50, 0.05, 0.52, 0.17)
+        r.append(t.mac.astype(np.float64))
+    assert np.abs(r[0] - r[1]).max() < tol
 
 
 # ---- link classification (lbm.py:114-124, 185-188) -----------------------------------------

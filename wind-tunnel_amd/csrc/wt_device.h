@@ -63,13 +63,22 @@ constexpr uint8_t CLS_NEAR = 128; // interior fluid cell with a non-fluid neighb
 constexpr uint8_t CLS_FORCE = 64; // interior fluid cell with a static-solid neighbour
 
 // ---- storage forms ----------------------------------------------------------
+// load/store: the population f_i of a stored element (weight added back for the
+// deviation forms). pop/put: the value the collision works on -- f_i for FP32
+// and FP64, the deviation g_i = f_i - w_i for FP16 and BF16, whose collision
+// runs on deviations throughout (dev = true). excess: f_i - w_i of a stored
+// element, the momentum-exchange term of lbm.py:189.
 template <int FORM> struct Form;
 
 template <> struct Form<WT_FP32> {
   typedef float elem;
   typedef float real;
+  static constexpr bool dev = false;
   __device__ static float load(elem e, int) { return e; }
   __device__ static elem store(float f, int) { return f; }
+  __device__ static float pop(elem e) { return e; }
+  __device__ static elem put(float v) { return v; }
+  __device__ static float excess(elem e, int i) { return e - (float)wd(i); }
 };
 
 __device__ inline float h2f(uint16_t b) { return (float)__builtin_bit_cast(_Float16, b); }
@@ -83,22 +92,34 @@ __device__ inline uint16_t f2bf(float f) { return __builtin_bit_cast(uint16_t, (
 template <> struct Form<WT_FP16> {
   typedef uint16_t elem;
   typedef float real;
+  static constexpr bool dev = true;
   __device__ static float load(elem e, int i) { return (float)wd(i) + h2f(e); }
   __device__ static elem store(float f, int i) { return f2h(f - (float)wd(i)); }
+  __device__ static float pop(elem e) { return h2f(e); }
+  __device__ static elem put(float v) { return f2h(v); }
+  __device__ static float excess(elem e, int) { return h2f(e); }
 };
 
 template <> struct Form<WT_BF16> {
   typedef uint16_t elem;
   typedef float real;
+  static constexpr bool dev = true;
   __device__ static float load(elem e, int i) { return (float)wd(i) + bf2f(e); }
   __device__ static elem store(float f, int i) { return f2bf(f - (float)wd(i)); }
+  __device__ static float pop(elem e) { return bf2f(e); }
+  __device__ static elem put(float v) { return f2bf(v); }
+  __device__ static float excess(elem e, int) { return bf2f(e); }
 };
 
 template <> struct Form<WT_FP64> {
   typedef double elem;
   typedef double real;
+  static constexpr bool dev = false;
   __device__ static double load(elem e, int) { return e; }
   __device__ static elem store(double f, int) { return f; }
+  __device__ static double pop(elem e) { return e; }
+  __device__ static elem put(double v) { return v; }
+  __device__ static double excess(elem e, int i) { return e - wd(i); }
 };
 
 inline int form_elem_bytes(int form) { return form == WT_FP32 ? 4 : (form == WT_FP64 ? 8 : 2); }
@@ -140,6 +161,7 @@ struct StepDesc {
   uint32_t ntiles;
   int32_t flags;
   int32_t global_addr; // 1: populations beyond 32-bit buffer offsets, use 64-bit global addresses
+  int32_t cpt;         // cells per thread of this tunnel's tiles (batched launches: 1, 2 or 4)
 };
 
 // Shell layout: six layers next to the faces, 4 reals per entry.

@@ -189,6 +189,134 @@ struct Acc {
 };
 
 // ---- collision (lbm.py:125-165) --------------------------------------------------------
+// Velocity clamp |u| <= 0.35 (lbm.py:130-135); returns ux^2 + uy^2 + uz^2 of the
+// clamped velocity (usq = 1.5 times it, lbm.py:136). The square root is taken only
+// where ux^2 + uy^2 + uz^2 > 0.12: correctly rounded sqrt is monotonic and
+// sqrt(0.12) rounds to 0.3464 < 0.35, so below that no lane can clamp and the
+// result equals the unconditional test of the reference bit for bit (NaN fails
+// both comparisons alike).
+template <typename R>
+__device__ __forceinline__ R clamp_velocity(R& ux, R& uy, R& uz) {
+  R u2 = ux * ux + uy * uy + uz * uz;
+  if (u2 > (R)0.12) {
+    const R um = sqrt(u2);
+    if (um > (R)0.35) {
+      const R sc = (R)0.35 / um;
+      ux = ux * sc; uy = uy * sc; uz = uz * sc;
+      u2 = ux * ux + uy * uy + uz * uz;
+    }
+  }
+  return u2;
+}
+
+// Signed sum over the directions with a nonzero coefficient k(i), in index order,
+// starting from the first such term (whose coefficient is +1 for every moment
+// used below): the deviation form's sums, which carry no leading 0.0.
+template <typename R, typename K>
+__device__ __forceinline__ R signed_sum(const R (&v)[Q], K k) {
+  R acc = (R)0;
+  bool any = false;
+#pragma unroll
+  for (int i = 0; i < Q; i++) {
+    const int c = k(i);
+    if (c == 0) continue;
+    if (!any) { acc = c > 0 ? v[i] : -v[i]; any = true; }
+    else if (c > 0) acc = acc + v[i];
+    else acc = acc - v[i];
+  }
+  return acc;
+}
+
+// Deviation-form collision (FP16 and BF16 storage): the same regularized
+// Smagorinsky collision as collide() below, carried out on g_i = f_i - w_i in
+// FP32, so no weight is added after the load or subtracted before the store.
+//   dr = sum g_i, rho = 1 + dr, (jx, jy, jz) = sum c_i g_i, u = j / rho (clamped);
+//   excess equilibrium eps_i = e_i - w_i = w_i dr + (w_i rho)(cu + (h - usq)),
+//   cu = 3 c_i.u, h = (0.5 cu) cu;  d_i = g_i - eps_i;  Pi = sum c c d;
+//   g_out_i = eps_i + 4.5 w_i keep (c_i c_i : Pi - tr Pi / 3).
+// Opposite directions share h and give cu_{i+1} = -cu_i exactly, so the pair
+// form below equals a per-direction evaluation. The oracle restates this
+// sequence operation by operation for these forms.
+__device__ __forceinline__ void collide_dev(float (&g)[Q], int x, int nx, float tau, float smag, float (&out)[Q],
+                                            float& ux, float& uy, float& uz, float& rho) {
+  float dr = g[0];
+#pragma unroll
+  for (int i = 1; i < Q; i++) dr = dr + g[i];
+  rho = 1.0f + dr;
+  const float ir = 1.0f / rho;
+  const float jx = signed_sum(g, [](int i) { return cx(i); });
+  const float jy = signed_sum(g, [](int i) { return cy(i); });
+  const float jz = signed_sum(g, [](int i) { return cz(i); });
+  ux = jx * ir; uy = jy * ir; uz = jz * ir;
+  const float usq = 1.5f * clamp_velocity<float>(ux, uy, uz);
+  const float w0 = (float)wd(0), w1 = (float)wd(1), w2 = (float)wd(7);
+  const float a0 = w0 * dr, a1 = w1 * dr, a2 = w2 * dr;
+  const float b0 = w0 * rho, b1 = w1 * rho, b2 = w2 * rho;
+  float eps[Q];
+  eps[0] = a0 - b0 * usq;
+#pragma unroll
+  for (int i = 1; i < Q; i += 2) {
+    // c_i for odd i has a first nonzero component of +1
+    float sdot;
+    if (cx(i) != 0) {
+      sdot = ux;
+      if (cy(i) > 0) sdot = sdot + uy; else if (cy(i) < 0) sdot = sdot - uy;
+      if (cz(i) > 0) sdot = sdot + uz; else if (cz(i) < 0) sdot = sdot - uz;
+    } else if (cy(i) != 0) {
+      sdot = uy;
+      if (cz(i) > 0) sdot = sdot + uz; else if (cz(i) < 0) sdot = sdot - uz;
+    } else {
+      sdot = uz;
+    }
+    const float cu = 3.0f * sdot;
+    const float h = 0.5f * cu * cu;
+    const float hu = h - usq;
+    const float a = i <= 6 ? a1 : a2, b = i <= 6 ? b1 : b2;
+    eps[i] = a + b * (cu + hu);
+    eps[i + 1] = a + b * (hu - cu);
+  }
+  float d[Q];
+#pragma unroll
+  for (int i = 0; i < Q; i++) d[i] = g[i] - eps[i];
+  const float pxx = signed_sum(d, [](int i) { return cx(i) * cx(i); });
+  const float pyy = signed_sum(d, [](int i) { return cy(i) * cy(i); });
+  const float pzz = signed_sum(d, [](int i) { return cz(i) * cz(i); });
+  const float pxy = signed_sum(d, [](int i) { return cx(i) * cy(i); });
+  const float pxz = signed_sum(d, [](int i) { return cx(i) * cz(i); });
+  const float pyz = signed_sum(d, [](int i) { return cy(i) * cz(i); });
+  const float qn = sqrt(pxx * pxx + pyy * pyy + pzz * pzz + 2.0f * (pxy * pxy + pxz * pxz + pyz * pyz));
+  float tau0 = tau;
+  const float sp = (float)x - (float)nx + 24.0f;
+  if (sp > 0.0f) tau0 = tau0 + sp * 0.015f;
+  const float tau_e = 0.5f * (tau0 + sqrt(tau0 * tau0 + 25.4558441f * smag * smag * qn * ir));
+  const float keep = 1.0f - 1.0f / tau_e;
+  const float tr3 = (pxx + pyy + pzz) * (1.0f / 3.0f);
+  const float k0 = (float)(4.5 * wd(0)) * keep, k1 = (float)(4.5 * wd(1)) * keep, k2 = (float)(4.5 * wd(7)) * keep;
+  out[0] = eps[0] - k0 * tr3;
+#pragma unroll
+  for (int i = 1; i < Q; i += 2) {
+    float acc2 = 0.0f;
+    bool any = false;
+    if (cx(i) != 0) { acc2 = pxx; any = true; }
+    if (cy(i) != 0) { acc2 = any ? acc2 + pyy : pyy; any = true; }
+    if (cz(i) != 0) { acc2 = any ? acc2 + pzz : pzz; any = true; }
+    if (cx(i) * cy(i) != 0) { const float t = (float)(2 * cx(i) * cy(i)) * pxy; acc2 = any ? acc2 + t : t; any = true; }
+    if (cx(i) * cz(i) != 0) { const float t = (float)(2 * cx(i) * cz(i)) * pxz; acc2 = any ? acc2 + t : t; any = true; }
+    if (cy(i) * cz(i) != 0) { const float t = (float)(2 * cy(i) * cz(i)) * pyz; acc2 = any ? acc2 + t : t; any = true; }
+    const float gk = (i <= 6 ? k1 : k2) * (acc2 - tr3);
+    out[i] = eps[i] + gk;
+    out[i + 1] = eps[i + 1] + gk;
+  }
+}
+
+// Collision of a storage form: deviation form for FP16/BF16, the reference's
+// order of operations (collide) for FP32 and FP64.
+template <int FORM>
+__device__ __forceinline__ void collide_form(typename Form<FORM>::real (&f)[Q], int x, int nx,
+                                             typename Form<FORM>::real tau, typename Form<FORM>::real smag,
+                                             typename Form<FORM>::real (&out)[Q], typename Form<FORM>::real& ux,
+                                             typename Form<FORM>::real& uy, typename Form<FORM>::real& uz,
+                                             typename Form<FORM>::real& rho);
 // Moments, velocity clamp, regularized collision with Smagorinsky LES and the
 // outlet sponge, for populations f after streaming; writes post-collision
 // populations to out. Opposite directions (i, i+1 for odd i) share their
@@ -211,12 +339,7 @@ __device__ __forceinline__ void collide(R (&f)[Q], int x, int nx, R tau, R smag,
 #pragma unroll
   for (int i = 0; i < Q; i++) { if (cz(i) > 0) sz = sz + f[i]; else if (cz(i) < 0) sz = sz - f[i]; }
   ux = sx * ir; uy = sy * ir; uz = sz * ir;
-  const R um = sqrt(ux * ux + uy * uy + uz * uz);
-  if (um > (R)0.35) {
-    const R sc = (R)0.35 / um;
-    ux = ux * sc; uy = uy * sc; uz = uz * sc;
-  }
-  const R usq = (R)1.5 * (ux * ux + uy * uy + uz * uz);
+  const R usq = (R)1.5 * clamp_velocity<R>(ux, uy, uz);
   const R wr0 = (R)wd(0) * rho, wr1 = (R)wd(1) * rho, wr2 = (R)wd(7) * rho;
   // equilibria: e_i = (w rho) * (((1 + cu) + (0.5 cu) cu) - usq)
   R e[Q];
@@ -268,6 +391,16 @@ __device__ __forceinline__ void collide(R (&f)[Q], int x, int nx, R tau, R smag,
   }
 }
 
+template <int FORM>
+__device__ __forceinline__ void collide_form(typename Form<FORM>::real (&f)[Q], int x, int nx,
+                                             typename Form<FORM>::real tau, typename Form<FORM>::real smag,
+                                             typename Form<FORM>::real (&out)[Q], typename Form<FORM>::real& ux,
+                                             typename Form<FORM>::real& uy, typename Form<FORM>::real& uz,
+                                             typename Form<FORM>::real& rho) {
+  if constexpr (Form<FORM>::dev) collide_dev(f, x, nx, tau, smag, out, ux, uy, uz, rho);
+  else collide<typename Form<FORM>::real>(f, x, nx, tau, smag, out, ux, uy, uz, rho);
+}
+
 // Momentum exchange of one interior cell on the populations it stores
 // (lbm.py:185-193): links i whose downstream neighbour is a static solid.
 template <int FORM>
@@ -279,7 +412,7 @@ __device__ __forceinline__ void add_force(const uint8_t* cls, uint32_t idx, int3
 #pragma unroll
   for (int i = 1; i < Q; i++) {
     if ((fl >> i) & 1u) {
-      const R v = (R)2.0 * (S::load(st[i], i) - (R)wd(i));
+      const R v = (R)2.0 * S::excess(st[i], i);
       const double dv = (double)v;
       if (cx(i) > 0) acc.fx += dv; else if (cx(i) < 0) acc.fx -= dv;
       if (cy(i) > 0) acc.fy += dv; else if (cy(i) < 0) acc.fy -= dv;
@@ -426,7 +559,7 @@ __device__ __forceinline__ void step_cell_general(const StepDesc& d, const typen
       const int i = LOAD_ORDER[k];
       const bool b = (lk.bb >> i) & 1u;
       const uint32_t a = pc + (b ? (uint32_t)ppos(opp(i)) * (uint32_t)nx : pshift(i, nx, nxy));
-      f[i] = S::load(ld_stream(fin + a), i);
+      f[i] = S::pop(ld_stream(fin + a));
     }
   };
   if constexpr (sizeof(E) <= 4) {
@@ -443,7 +576,7 @@ __device__ __forceinline__ void step_cell_general(const StepDesc& d, const typen
         const uint32_t pull = pshift(i, nx, nxy) * ES + bias;
         const uint32_t wall = (uint32_t)ppos(opp(i)) * (uint32_t)nx * ES + bias;
         const uint32_t m = 0u - ((lk.bb >> i) & 1u);
-        f[i] = S::load(RawIO<E>::ld(rin, bo + (pull + (m & (wall - pull))), 0u, POL == 1 && cx(i) == 0), i);
+        f[i] = S::pop(RawIO<E>::ld(rin, bo + (pull + (m & (wall - pull))), 0u, POL == 1 && cx(i) == 0));
       }
     } else {
       global_loads();
@@ -458,10 +591,10 @@ __device__ __forceinline__ void step_cell_general(const StepDesc& d, const typen
       if (cx(i) != 0 && ((lk.belt >> i) & 1u)) f[i] = f[i] + (R)(6.0 * wd(i) * cx(i)) * uin;
   }
   R out[Q], ux, uy, uz, rho;
-  collide<R>(f, x, nx, (R)d.tau, (R)d.smag, out, ux, uy, uz, rho);
+  collide_form<FORM>(f, x, nx, (R)d.tau, (R)d.smag, out, ux, uy, uz, rho);
   E st[Q];
 #pragma unroll
-  for (int i = 0; i < Q; i++) st[i] = S::store(out[i], i);
+  for (int i = 0; i < Q; i++) st[i] = S::put(out[i]);
   store_all(st);
   st_mac<POL == 1>(d.mac + idx, (float)ux, (float)uy, (float)uz, (float)rho);
   shell_write<R>(shout, x, y, z, ux, uy, uz, rho);
@@ -493,15 +626,18 @@ __device__ __forceinline__ void step_cell_fast(const StepDesc& d, const typename
     // inlet / outlet lanes of this wave (lbm.py:84-109)
     face_state<R>(d, shin, x, y, z, rho, ux, uy, uz);
     const R bsq = (R)1.5 * (ux * ux + uy * uy + uz * uz);
+    // in the deviation forms the common store below takes g = f - w: the same
+    // subtraction store() makes on the face path
 #pragma unroll
-    for (int i = 0; i < Q; i++) out[i] = face_feq<R>(i, rho, ux, uy, uz, bsq);
+    for (int i = 0; i < Q; i++)
+      out[i] = S::dev ? face_feq<R>(i, rho, ux, uy, uz, bsq) - (R)wd(i) : face_feq<R>(i, rho, ux, uy, uz, bsq);
   } else {
     R f[Q];
     auto global_loads = [&]() {
 #pragma unroll
       for (int k = 0; k < Q; k++) {
         const int i = LOAD_ORDER[k];
-        f[i] = S::load(ld_stream(fin + (pc + pshift(i, nx, nxy))), i);
+        f[i] = S::pop(ld_stream(fin + (pc + pshift(i, nx, nxy))));
       }
     };
     if constexpr (sizeof(E) <= 4) {
@@ -511,7 +647,7 @@ __device__ __forceinline__ void step_cell_fast(const StepDesc& d, const typename
 #pragma unroll
         for (int k = 0; k < Q; k++) {
           const int i = LOAD_ORDER[k];
-          f[i] = S::load(RawIO<E>::ld(rin, bo, pshift(i, nx, nxy) * ES + bias, POL == 1 && cx(i) == 0), i);
+          f[i] = S::pop(RawIO<E>::ld(rin, bo, pshift(i, nx, nxy) * ES + bias, POL == 1 && cx(i) == 0));
         }
       } else {
         global_loads();
@@ -519,18 +655,18 @@ __device__ __forceinline__ void step_cell_fast(const StepDesc& d, const typename
     } else {
       global_loads();
     }
-    collide<R>(f, x, nx, (R)d.tau, (R)d.smag, out, ux, uy, uz, rho);
+    collide_form<FORM>(f, x, nx, (R)d.tau, (R)d.smag, out, ux, uy, uz, rho);
   }
   auto global_stores = [&]() {
 #pragma unroll
-    for (int i = 0; i < Q; i++) st_stream<POL != 2>(fout + (pc + (uint32_t)ppos(i) * nx), S::store(out[i], i));
+    for (int i = 0; i < Q; i++) st_stream<POL != 2>(fout + (pc + (uint32_t)ppos(i) * nx), S::put(out[i]));
   };
   if constexpr (sizeof(E) <= 4) {
     if (!d.global_addr) {
       const __amdgpu_buffer_rsrc_t rout =
           __builtin_amdgcn_make_buffer_rsrc((void*)fout, (short)0, (int)((uint32_t)Q * n * ES), BUF_GFX9_WORD3);
 #pragma unroll
-      for (int i = 0; i < Q; i++) RawIO<E>::st(rout, bo, (uint32_t)ppos(i) * nx * ES, S::store(out[i], i), POL != 2);
+      for (int i = 0; i < Q; i++) RawIO<E>::st(rout, bo, (uint32_t)ppos(i) * nx * ES, S::put(out[i]), POL != 2);
     } else {
       global_stores();
     }
@@ -670,17 +806,39 @@ __global__ __launch_bounds__(BLOCK) WT_STEP_ATTR void k_step_adv(StepDesc d, Adv
   }
 }
 
-template <int FORM, int CPT>
-__global__ __launch_bounds__(BLOCK) void k_step_batch(const StepDesc* __restrict__ ds, int nd,
-                                                      int parity, int remap, int flags) {
-  const uint32_t tile = xcd_tile(blockIdx.x, gridDim.x, remap);
-  int lo = 0, hi = nd - 1;
-  while (lo < hi) {
-    const int mid = (lo + hi + 1) >> 1;
-    if (ds[mid].tile_begin <= tile) lo = mid; else hi = mid - 1;
+// Tunnel k of a batch whose items [tile_begin, tile_begin + ntiles) hold
+// `item`, searched from the previous item's tunnel: a workgroup walks the items
+// in order (or in reverse), so this moves by at most a step or two.
+__device__ __forceinline__ int find_tunnel(const StepDesc* __restrict__ ds, int nd, int k, uint32_t item) {
+  while (k + 1 < nd && ds[k + 1].tile_begin <= item) k++;
+  while (k > 0 && ds[k].tile_begin > item) k--;
+  return k;
+}
+
+// Persistent batched step: the gridDim.x workgroups (about one full occupancy of
+// the GPU) stride over the work items, tile t of tunnel k for every tunnel of the
+// batch, tunnel k's tiles holding 256 * ds[k].cpt cells (the launch shape of its
+// size class). At any moment the running items form one contiguous front, as in a
+// launch of one workgroup per tile. With `backwards` the front runs from the last
+// item to the first. Each cell is computed exactly as by k_step.
+template <int FORM>
+__global__ __launch_bounds__(BLOCK) void k_step_batch(const StepDesc* __restrict__ ds, int nd, uint32_t nitems,
+                                                      int parity, int backwards, int flags) {
+  int k = backwards ? nd - 1 : 0;
+  for (uint32_t it = blockIdx.x; it < nitems; it += gridDim.x) {
+    const uint32_t item = backwards ? nitems - 1u - it : it;
+    k = find_tunnel(ds, nd, k, item);
+    const StepDesc d = ds[k];
+    const int p = parity ^ (d.flags & 1);
+    const uint32_t tile = item - d.tile_begin;
+    switch (d.cpt) {
+      case 2: run_tile<FORM, 2, 0>(d, p, tile, flags); break;
+      case 4: run_tile<FORM, 4, 0>(d, p, tile, flags); break;
+      default: run_tile<FORM, 1, 0>(d, p, tile, flags); break;
+    }
+    // the next item's slot stage rewrites the LDS this item's flush reads
+    if (flags & (WT_FLAG_FORCES | WT_FLAG_MASS)) __syncthreads();
   }
-  const StepDesc d = ds[lo];
-  run_tile<FORM, CPT, 0>(d, parity ^ (d.flags & 1), tile - d.tile_begin, flags);
 }
 
 #ifndef WT_STEP_FP32_UNIT  // kernels launched only from the main unit
@@ -792,7 +950,7 @@ __global__ __launch_bounds__(BLOCK) void k_force(StepDesc d) {
 #pragma unroll
       for (int i = 1; i < Q; i++) {
         if ((fl >> i) & 1u) {
-          const R v = (R)2.0 * (S::load(f[pc + (uint32_t)ppos(i) * (uint32_t)nx], i) - (R)wd(i));
+          const R v = (R)2.0 * S::excess(f[pc + (uint32_t)ppos(i) * (uint32_t)nx], i);
           const double dv = (double)v;
           if (cx(i) > 0) a.fx += dv; else if (cx(i) < 0) a.fx -= dv;
           if (cy(i) > 0) a.fy += dv; else if (cy(i) < 0) a.fy -= dv;
@@ -943,20 +1101,21 @@ void launch_step_adv_f(const StepDesc& d, const AdvectArgs& a, Shape sh, int fla
   }
 }
 
-template <int FORM, int CPT>
-static void launch_batch_t(const StepDesc* dd, int nd, uint32_t ntile, int parity, int remap,
-                           int flags, hipStream_t s) {
-  hipLaunchKernelGGL((k_step_batch<FORM, CPT>), dim3(ntile), dim3(BLOCK), 0, s, dd, nd, parity, remap,
+template <int FORM>
+void launch_batch_f(const StepDesc* dd, int nd, uint32_t nitems, uint32_t blocks, int parity, int backwards,
+                    int flags, hipStream_t s) {
+  hipLaunchKernelGGL((k_step_batch<FORM>), dim3(blocks), dim3(BLOCK), 0, s, dd, nd, nitems, parity, backwards,
                      flags);
 }
+// Workgroups of one full occupancy of the batched kernel on the current device.
 template <int FORM>
-void launch_batch_f(const StepDesc* dd, int nd, uint32_t ntile, int parity, int cpt, int remap,
-                           int flags, hipStream_t s) {
-  switch (cpt) {
-    case 2: launch_batch_t<FORM, 2>(dd, nd, ntile, parity, remap, flags, s); break;
-    case 4: launch_batch_t<FORM, 4>(dd, nd, ntile, parity, remap, flags, s); break;
-    default: launch_batch_t<FORM, 1>(dd, nd, ntile, parity, remap, flags, s); break;
-  }
+int batch_blocks_f() {
+  int dev = 0, ncu = 0, per_cu = 0;
+  if (hipGetDevice(&dev) != hipSuccess) return -1;
+  if (hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess) return -1;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void*)k_step_batch<FORM>, BLOCK, 0) != hipSuccess)
+    return -1;
+  return ncu * std::max(per_cu, 1);
 }
 
 // The FP32 step kernels are instantiated in their own translation unit,
@@ -968,16 +1127,18 @@ void launch_batch_f(const StepDesc* dd, int nd, uint32_t ntile, int parity, int 
 template void launch_step_f<WT_FP32>(const StepDesc& d, int parity, Shape sh, int flags, hipStream_t s);
 template void launch_step_adv_f<WT_FP32>(const StepDesc& d, const AdvectArgs& a, Shape sh, int flags,
                                         hipStream_t s);
-template void launch_batch_f<WT_FP32>(const StepDesc* dd, int nd, uint32_t ntile, int parity, int cpt, int remap,
-                                      int flags, hipStream_t s);
+template void launch_batch_f<WT_FP32>(const StepDesc* dd, int nd, uint32_t nitems, uint32_t blocks, int parity,
+                                      int backwards, int flags, hipStream_t s);
+template int batch_blocks_f<WT_FP32>();
 
 }  // namespace wt
 #else
 extern template void launch_step_f<WT_FP32>(const StepDesc& d, int parity, Shape sh, int flags, hipStream_t s);
 extern template void launch_step_adv_f<WT_FP32>(const StepDesc& d, const AdvectArgs& a, Shape sh, int flags,
                                                hipStream_t s);
-extern template void launch_batch_f<WT_FP32>(const StepDesc* dd, int nd, uint32_t ntile, int parity, int cpt,
-                                             int remap, int flags, hipStream_t s);
+extern template void launch_batch_f<WT_FP32>(const StepDesc* dd, int nd, uint32_t nitems, uint32_t blocks,
+                                             int parity, int backwards, int flags, hipStream_t s);
+extern template int batch_blocks_f<WT_FP32>();
 
 static int64_t nslots_of(int64_t n) { return (n + SLOT - 1) / SLOT; }
 static int64_t nchunks_of(int64_t n) { return (nslots_of(n) + CHUNK - 1) / CHUNK; }
@@ -1030,7 +1191,16 @@ static int reduce_one(const wt_grid* g, hipStream_t s) {
 }
 
 
-static Shape g_shape[4] = {{1, 0, 0}, {1, 0, 0}, {1, 0, 0}, {1, 0, 0}};
+// Launch shape of a tunnel's step kernels from its grid (cpt 0 means 1).
+static int grid_shape(const wt_grid* g, Shape* sh) {
+  const int cpt = g->cpt == 0 ? 1 : g->cpt;
+  if (cpt != 1 && cpt != 2 && cpt != 4) return fail(-1, "cells per thread must be 1, 2 or 4 (got %d)", g->cpt);
+  if (g->tile_order & ~7) return fail(-1, "unknown tile order bits 0x%x", g->tile_order);
+  sh->cpt = cpt;
+  sh->remap = g->tile_order;
+  sh->solo = 0;
+  return 0;
+}
 
 // Cache policy of a single-tunnel launch (run_tile's POL) by the bytes of its two
 // population buffers. Streaming (1) pays only far above the 256 MB Infinity Cache:
@@ -1062,18 +1232,11 @@ using namespace wt;
 
 extern "C" {
 
-int wt_set_launch_shape(int form, int cpt, int remap) {
-  if (form < WT_FP32 || form > WT_FP64) return fail(-1, "unknown storage form %d", form);
-  if (cpt != 1 && cpt != 2 && cpt != 4) return fail(-1, "cells per thread must be 1, 2 or 4 (got %d)", cpt);
-  g_shape[form] = Shape{cpt, remap & 7};
-  return 0;
-}
-
-int wt_get_launch_shape(int form, int* cpt, int* remap) {
-  if (form < WT_FP32 || form > WT_FP64) return fail(-1, "unknown storage form %d", form);
-  if (cpt) *cpt = g_shape[form].cpt;
-  if (remap) *remap = g_shape[form].remap;
-  return 0;
+int wt_cache_policy(int nx, int ny, int nz, int form) {
+  wt_grid g{};
+  g.nx = nx; g.ny = ny; g.nz = nz; g.form = form;
+  if (int e = check_grid(&g, false)) return e;
+  return solo_policy(&g);
 }
 
 int wt_buffer_sizes(int nx, int ny, int nz, int form, int64_t sizes[6]) {
@@ -1142,8 +1305,9 @@ int wt_step(const wt_grid* g, int cur, int nsteps, const wt_params* p, int flags
   if (nsteps < 0) return fail(-1, "negative step count %d", nsteps);
   if ((flags & (WT_FLAG_FORCES | WT_FLAG_MASS)) && (!g->partials || !g->sums))
     return fail(-1, "reduction flags need partials and sums buffers");
+  Shape sh;
+  if (int e = grid_shape(g, &sh)) return e;
   hipStream_t s = (hipStream_t)stream;
-  const Shape sh = g_shape[g->form];
   const int solo = solo_policy(g);
   StepDesc d = make_desc(g, cur, p, 0);
   if (sh.remap & 4) d.global_addr = 1;  // shape bit 2: 64-bit global addressing at any size
@@ -1173,12 +1337,13 @@ int wt_step_advect(const wt_grid* g, int cur, const wt_params* p, int flags, con
   if (num_p > 0 && (!pos || !spd || !box || !mac_prev)) return fail(-1, "null particle buffer, field or seed box");
   if (num_p > 0 && (const void*)mac_prev == (const void*)g->mac)
     return fail(-1, "the advected field must not be the grid's mac buffer (the step writes it)");
+  Shape sh;
+  if (int e = grid_shape(g, &sh)) return e;
   hipStream_t s = (hipStream_t)stream;
-  Shape sh = g_shape[g->form];
+  StepDesc d = make_desc(g, cur, p, 0);
+  if (sh.remap & 4) d.global_addr = 1;
   sh.remap = (sh.remap & 1) | (((sh.remap & 2) && (cur & 1)) ? 2 : 0);   // as step k = 0 of wt_step
   sh.solo = solo_policy(g);
-  StepDesc d = make_desc(g, cur, p, 0);
-  if (g_shape[g->form].remap & 4) d.global_addr = 1;
   const wt_seed_box none{0.f, 0.f, 0.f, 0.f, 0.f};
   const AdvectArgs a{reinterpret_cast<const float4*>(mac_prev), g->cls, nullptr, ids, reinterpret_cast<float4*>(pos),
                      spd, (uint32_t)num_p, g->nx, g->ny, g->nz, dt, uin, max_age, seed, box ? *box : none};
@@ -1245,7 +1410,6 @@ int wt_batch_prepare(const wt_grid* grids, const wt_params* params, const int* c
     return fail(-1, "null argument or empty batch");
   const int form = grids[0].form;
   if (form < WT_FP32 || form > WT_FP64) return fail(-1, "unknown storage form %d", form);
-  const Shape sh = g_shape[form];
   std::vector<StepDesc> ds(ntunnels);
   std::vector<RedDesc> rs(ntunnels);
   uint64_t tile = 0;
@@ -1255,7 +1419,11 @@ int wt_batch_prepare(const wt_grid* grids, const wt_params* params, const int* c
     if (grids[k].form != form)
       return fail(-1, "tunnel %d has storage form %d, batch uses %d", k, grids[k].form, form);
     if (!grids[k].partials || !grids[k].sums) return fail(-1, "tunnel %d has no reduction buffers", k);
+    Shape sh;
+    if (int e = grid_shape(&grids[k], &sh)) return e;
     StepDesc d = make_desc(&grids[k], 0, &params[k], 0);
+    if (sh.remap & 4) d.global_addr = 1;
+    d.cpt = sh.cpt;
     d.tile_begin = (uint32_t)tile;
     d.ntiles = (d.n + BLOCK * sh.cpt - 1) / (BLOCK * sh.cpt);
     d.flags = cur[k] & 1;
@@ -1269,6 +1437,14 @@ int wt_batch_prepare(const wt_grid* grids, const wt_params* params, const int* c
       rs[k] = RedDesc{grids[k].partials, grids[k].sums, grids[k].sums, (uint32_t)nslots_of(d.n)};
     max_chunks = std::max<int64_t>(max_chunks, nchunks_of(d.n));
   }
+  int blocks = -1;
+  switch (form) {
+    case WT_FP32: blocks = batch_blocks_f<WT_FP32>(); break;
+    case WT_FP16: blocks = batch_blocks_f<WT_FP16>(); break;
+    case WT_BF16: blocks = batch_blocks_f<WT_BF16>(); break;
+    case WT_FP64: blocks = batch_blocks_f<WT_FP64>(); break;
+  }
+  if (blocks <= 0) return fail(-2, "occupancy query of the batched step failed: %s", hipGetErrorString(hipGetLastError()));
   hipStream_t s = (hipStream_t)stream;
   char* ws = (char*)workspace;
   if (hipMemcpyAsync(ws, ds.data(), sizeof(StepDesc) * ntunnels, hipMemcpyHostToDevice, s) != hipSuccess ||
@@ -1281,8 +1457,9 @@ int wt_batch_prepare(const wt_grid* grids, const wt_params* params, const int* c
   info->max_chunks = (int32_t)max_chunks;
   info->form = form;
   info->ntunnels = ntunnels;
-  info->cpt = sh.cpt;
-  info->remap = sh.remap;
+  info->blocks = (int32_t)std::min<int64_t>(blocks, (int64_t)tile);
+  info->tile_order = 2;
+  info->reserved = 0;
   return 0;
 }
 
@@ -1293,18 +1470,21 @@ int wt_batch_step(void* workspace, const wt_batch_info* info, int parity, int fl
   if (!workspace || !info || info->ntunnels <= 0 || info->total_tiles <= 0) return fail(-1, "empty batch");
   const int form = info->form;
   if (form < WT_FP32 || form > WT_FP64) return fail(-1, "unknown storage form %d", form);
+  if (info->blocks <= 0) return fail(-1, "batch launch needs at least one workgroup (got %d)", info->blocks);
+  if (info->tile_order & ~2) return fail(-1, "unknown batch tile order bits 0x%x", info->tile_order);
   hipStream_t s = (hipStream_t)stream;
   const int nt = info->ntunnels;
   const StepDesc* dd = (const StepDesc*)workspace;
   const RedDesc* rd = (const RedDesc*)((const char*)workspace + sizeof(StepDesc) * nt);
   const int f = flags & (WT_FLAG_FORCES | WT_FLAG_MASS);
-  const uint32_t ntile = (uint32_t)info->total_tiles;
-  const int remap = (info->remap & 1) | (((info->remap & 2) && (parity & 1)) ? 2 : 0);
+  const uint32_t nitems = (uint32_t)info->total_tiles;
+  const uint32_t blocks = (uint32_t)std::min<int64_t>(info->blocks, info->total_tiles);
+  const int backwards = ((info->tile_order & 2) && (parity & 1)) ? 1 : 0;
   switch (form) {
-    case WT_FP32: launch_batch_f<WT_FP32>(dd, nt, ntile, parity, info->cpt, remap, f, s); break;
-    case WT_FP16: launch_batch_f<WT_FP16>(dd, nt, ntile, parity, info->cpt, remap, f, s); break;
-    case WT_BF16: launch_batch_f<WT_BF16>(dd, nt, ntile, parity, info->cpt, remap, f, s); break;
-    case WT_FP64: launch_batch_f<WT_FP64>(dd, nt, ntile, parity, info->cpt, remap, f, s); break;
+    case WT_FP32: launch_batch_f<WT_FP32>(dd, nt, nitems, blocks, parity, backwards, f, s); break;
+    case WT_FP16: launch_batch_f<WT_FP16>(dd, nt, nitems, blocks, parity, backwards, f, s); break;
+    case WT_BF16: launch_batch_f<WT_BF16>(dd, nt, nitems, blocks, parity, backwards, f, s); break;
+    case WT_FP64: launch_batch_f<WT_FP64>(dd, nt, nitems, blocks, parity, backwards, f, s); break;
   }
   if (f) {
     hipLaunchKernelGGL(k_reduce1, dim3((unsigned)info->max_chunks, (unsigned)nt), dim3(BLOCK), 0, s, rd);

@@ -25,7 +25,7 @@ c_i32, c_i64, c_u32, c_f32, c_vp = ctypes.c_int32, ctypes.c_int64, ctypes.c_uint
 
 class Grid(ctypes.Structure):
     """wt_grid"""
-    _fields_ = [('nx', c_i32), ('ny', c_i32), ('nz', c_i32), ('form', c_i32),
+    _fields_ = [('nx', c_i32), ('ny', c_i32), ('nz', c_i32), ('form', c_i32), ('cpt', c_i32), ('tile_order', c_i32),
                 ('f', c_vp * 2), ('cls', c_vp), ('mac', c_vp), ('shell', c_vp * 2),
                 ('partials', c_vp), ('sums', c_vp)]
 
@@ -43,7 +43,7 @@ class SeedBox(ctypes.Structure):
 class BatchInfo(ctypes.Structure):
     """wt_batch_info"""
     _fields_ = [('total_tiles', c_i64), ('max_chunks', c_i32), ('form', c_i32), ('ntunnels', c_i32),
-                ('cpt', c_i32), ('remap', c_i32), ('reserved', c_i32)]
+                ('blocks', c_i32), ('tile_order', c_i32), ('reserved', c_i32)]
 
 
 P = ctypes.POINTER
@@ -55,6 +55,7 @@ SIGNATURES = {
     'wt_buffer_sizes': (c_i32, [c_i32, c_i32, c_i32, c_i32, P(c_i64)]),
     'wt_state_bytes': (c_i64, [c_i32, c_i32, c_i32, c_i32]),
     'wt_tile_count': (c_i64, [c_i32, c_i32, c_i32]),
+    'wt_cache_policy': (c_i32, [c_i32, c_i32, c_i32, c_i32]),
     'wt_init': (c_i32, [P(Grid), c_f32, c_vp]),
     'wt_set_solid': (c_i32, [P(Grid), c_vp, c_vp]),
     'wt_step': (c_i32, [P(Grid), c_i32, c_i32, P(Params), c_i32, c_vp]),
@@ -65,8 +66,6 @@ SIGNATURES = {
     'wt_batch_workspace_bytes': (c_i64, [c_i32]),
     'wt_batch_prepare': (c_i32, [P(Grid), P(Params), P(c_i32), c_i32, c_vp, c_vp, P(BatchInfo), c_vp]),
     'wt_batch_step': (c_i32, [c_vp, P(BatchInfo), c_i32, c_i32, c_vp]),
-    'wt_set_launch_shape': (c_i32, [c_i32, c_i32, c_i32]),
-    'wt_get_launch_shape': (c_i32, [c_i32, P(c_i32), P(c_i32)]),
     'wt_advect': (c_i32, [P(Grid), c_vp, c_vp, c_i64, c_f32, c_f32, c_f32, c_u32, P(SeedBox), c_vp]),
     'wt_advect_ordered': (c_i32, [P(Grid), c_vp, c_vp, c_vp, c_i64, c_f32, c_f32, c_f32, c_u32, P(SeedBox), c_vp]),
     'wt_particle_order_workspace_bytes': (c_i64, [c_i64]),

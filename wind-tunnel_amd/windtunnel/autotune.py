@@ -59,12 +59,11 @@ def tune(forms, steps=40):
     import torch
     from . import geometry
     from .roofline import step_timing
-    from .tunnel import Tunnel, get_launch_shape, set_launch_shape
+    from .tunnel import Tunnel
     out = {'arch': torch.cuda.get_device_properties(0).gcnArchName, 'shapes': {}, 'timings': {}}
     for form in forms:
         out['shapes'][form] = {}
         out['timings'][form] = {}
-        old = get_launch_shape(form)
         for name, _, _, dims in CLASSES:
             ells = geometry.car_body(dims, 7, count=8,
                                      box=((0.2 * dims[0], 0.5 * dims[0]), (1, 0.3 * dims[1]),
@@ -76,8 +75,7 @@ def tune(forms, steps=40):
             t.set_solid(mask)
             best, res = None, {}
             for shape in SHAPES:
-                set_launch_shape(form, *shape)
-                _, r = step_timing(form, dims, mask, steps=steps, warmup=5, tunnel=t)
+                _, r = step_timing(form, dims, mask, steps=steps, warmup=5, tunnel=t, shape=shape)
                 res['%d,%d' % shape] = r['ms_median']
                 if best is None or r['ms_median'] < res['%d,%d' % best]:
                     best = shape
@@ -86,7 +84,6 @@ def tune(forms, steps=40):
             print(json.dumps({'form': form, 'class': name, 'dims': dims, 'best': best, 'ms': res}), flush=True)
             del t
             torch.cuda.empty_cache()
-        set_launch_shape(form, *old)
     return out
 
 

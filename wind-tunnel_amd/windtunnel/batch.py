@@ -1,5 +1,6 @@
-"""Many resident tunnels advanced together: one launch of the batched step
-kernel per lattice step for all tunnels of one storage form.
+"""Many resident tunnels advanced together: one persistent launch of the batched
+step kernel per lattice step for all tunnels of one storage form, each tunnel
+keeping the launch shape (cells per thread) of its size class.
 
 The descriptor table (per-tunnel buffers, sizes, parameters, tile offsets) is
 uploaded once per batch composition (wt_batch_prepare); each step is then a
@@ -11,9 +12,8 @@ import ctypes
 
 import torch
 
-from . import _lib, autotune
+from . import _lib
 from ._lib import BatchInfo, Grid, Params, check, stream_handle
-from .tunnel import apply_shape
 
 
 class Batch:
@@ -39,9 +39,8 @@ class Batch:
 
     def prepare(self, stream=None):
         nt = len(self.tunnels)
-        # the batch launches with the tuned shape of its largest tunnel's size class
-        big = max(self.tunnels, key=lambda t: t.n)
-        apply_shape(big.form, autotune.shape_for(big.form_name, big.n))
+        # each tunnel's grid carries the launch shape of its size class: its tiles
+        # hold 256 * cpt cells inside the one persistent launch
         grids = (Grid * nt)(*[t.grid for t in self.tunnels])
         prm = (Params * nt)(*[Params(*p) for p in self.params])
         cur = (ctypes.c_int32 * nt)(*[t.cur for t in self.tunnels])

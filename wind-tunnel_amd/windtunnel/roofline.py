@@ -23,7 +23,7 @@ import numpy as np
 import torch
 
 from . import _lib, autotune, geometry
-from .tunnel import Tunnel, get_launch_shape, set_launch_shape
+from .tunnel import Tunnel
 
 BYTES_PER_CELL = {'fp32': 169, 'fp16': 93, 'bf16': 93, 'fp64': 321}
 BYTES_PER_PARTICLE = 40
@@ -91,9 +91,10 @@ class _Rake:
         self.k += 1
 
 
-def step_timing(form, dims, mask, steps=100, warmup=10, flags=0, tunnel=None, particles=0):
+def step_timing(form, dims, mask, steps=100, warmup=10, flags=0, tunnel=None, particles=0, shape=None):
     t = tunnel or Tunnel(dims, form)
-    t.shape = None          # time whatever launch shape is currently set
+    if shape is not None:
+        t.shape = shape     # else the tunnel's own (the tuned shape of its size class)
     if tunnel is None:
         t.init(0.05)
         t.set_solid(mask)
@@ -120,7 +121,7 @@ def step_timing(form, dims, mask, steps=100, warmup=10, flags=0, tunnel=None, pa
     med = float(np.median(ms))
     nbytes = BYTES_PER_CELL[form] * n + BYTES_PER_PARTICLE * particles
     return t, {'what': 'step_advect' if particles else 'step', 'form': form, 'dims': list(dims),
-               'particles': particles, 'cpt_remap': list(get_launch_shape(form)),
+               'particles': particles, 'cpt_remap': list(t.shape),
                'flags': flags, 'ms_mean': float(ms.mean()), 'ms_median': med, 'ms_min': float(ms.min()),
                'mlups': n / (med * 1e-3) / 1e6, 'bytes_per_launch': nbytes,
                'gbs': nbytes / (med * 1e-3) / 1e9, 'frac': nbytes / (med * 1e-3) / 1e9 / PEAK_GBS}
@@ -149,13 +150,10 @@ def main(argv=None):
         if args.shapes:
             shapes = [tuple(int(v) for v in sh.split(',')) for sh in args.shapes.split()]
         t = None
-        old = get_launch_shape(form)
-        for cpt, remap in shapes:
-            set_launch_shape(form, cpt, remap)
+        for shape in shapes:
             np_ = args.particles if form in args.fused.split(',') else 0
-            t, r = step_timing(form, dims, mask, args.steps, tunnel=t, particles=np_)
+            t, r = step_timing(form, dims, mask, args.steps, tunnel=t, particles=np_, shape=shape)
             print(json.dumps(r), flush=True)
-        set_launch_shape(form, *old)
         del t
         torch.cuda.empty_cache()
     return 0

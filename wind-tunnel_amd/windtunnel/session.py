@@ -87,6 +87,11 @@ class Session:
         self.fuse = bool((form in FUSED_FORMS if fuse is None else fuse) and self.num_p)
         self.pending = False
         self.advected = 0             # steps whose particles have been advected
+        self._last_sort = None
+        if self.num_p:
+            # the first sort runs here, at set-up: its one-time cost (the sort library's
+            # first call, ~25 ms on the host) then falls outside every step
+            self._sort(None)
         if self.fuse:
             self.macs = (self.T.mac, self.T.mac.clone())
         # slice planes: (axis, index, field)
@@ -105,6 +110,13 @@ class Session:
         # frame's slices and the next force step wait for them (an in-stream event,
         # long satisfied by then). side_copies=False keeps them on the step's stream.
         self.copy_stream = torch.cuda.Stream(device=self.T.device) if side_copies else None
+        if self.copy_stream is not None:
+            # the first dispatch on a stream sets up its hardware queue: ~2 ms during
+            # which the host issues nothing (kernel traces show the GPU idle that long
+            # at a session's first frame), so make it here, at set-up
+            with torch.cuda.stream(self.copy_stream):
+                self.forces_host.copy_(self.T.sums, non_blocking=True)
+            self.copy_stream.synchronize()
         self._copied = None
         self.frames = 0
         self._averaged = 0
@@ -163,12 +175,16 @@ class Session:
 
     def _sort_if_due(self, stream):
         """Re-gather the particle slots in cell order every sort_every advections."""
-        if self.advected % self.sort_every == 0:
-            self.order_ws = self.T.particle_order(self.pos, self.order, self.order_ws, stream=stream)
-            pos2, spd2, ids2 = self._spare
-            self.T.particle_gather(self.pos, self.spd, self.ids, self.order, pos2, spd2, ids2, stream=stream)
-            self._spare = (self.pos, self.spd, self.ids)
-            self.pos, self.spd, self.ids = pos2, spd2, ids2
+        if self._last_sort is None or self.advected - self._last_sort >= self.sort_every:
+            self._sort(stream)
+
+    def _sort(self, stream):
+        self.order_ws = self.T.particle_order(self.pos, self.order, self.order_ws, stream=stream)
+        pos2, spd2, ids2 = self._spare
+        self.T.particle_gather(self.pos, self.spd, self.ids, self.order, pos2, spd2, ids2, stream=stream)
+        self._spare = (self.pos, self.spd, self.ids)
+        self.pos, self.spd, self.ids = pos2, spd2, ids2
+        self._last_sort = self.advected
 
     def _advect(self, stream=None):
         """Particle advection with dt = 1 on slot storage re-sorted by cell every sort_every steps."""

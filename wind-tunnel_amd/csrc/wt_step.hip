@@ -806,38 +806,30 @@ __global__ __launch_bounds__(BLOCK) WT_STEP_ATTR void k_step_adv(StepDesc d, Adv
   }
 }
 
-// Tunnel k of a batch whose items [tile_begin, tile_begin + ntiles) hold
-// `item`, searched from the previous item's tunnel: a workgroup walks the items
-// in order (or in reverse), so this moves by at most a step or two.
-__device__ __forceinline__ int find_tunnel(const StepDesc* __restrict__ ds, int nd, int k, uint32_t item) {
-  while (k + 1 < nd && ds[k + 1].tile_begin <= item) k++;
-  while (k > 0 && ds[k].tile_begin > item) k--;
-  return k;
-}
 
-// Persistent batched step: the gridDim.x workgroups (about one full occupancy of
-// the GPU) stride over the work items, tile t of tunnel k for every tunnel of the
-// batch, tunnel k's tiles holding 256 * ds[k].cpt cells (the launch shape of its
-// size class). At any moment the running items form one contiguous front, as in a
-// launch of one workgroup per tile. With `backwards` the front runs from the last
-// item to the first. Each cell is computed exactly as by k_step.
+// Batched step: one workgroup per work item, tile t of tunnel k for every tunnel
+// of the batch, tunnel k's tiles holding 256 * ds[k].cpt cells (the launch shape
+// of its size class, one unrolled code path per cells-per-thread value). With
+// `backwards` the items are dispatched from the last to the first. Each cell is
+// computed exactly as by k_step. (A persistent form, a grid of one occupancy
+// striding over the items, measured 16 % slower on 48 uneven tunnels: the loop
+// raised the kernel to 96 registers, 5 waves per SIMD instead of 7.)
 template <int FORM>
 __global__ __launch_bounds__(BLOCK) void k_step_batch(const StepDesc* __restrict__ ds, int nd, uint32_t nitems,
                                                       int parity, int backwards, int flags) {
-  int k = backwards ? nd - 1 : 0;
-  for (uint32_t it = blockIdx.x; it < nitems; it += gridDim.x) {
-    const uint32_t item = backwards ? nitems - 1u - it : it;
-    k = find_tunnel(ds, nd, k, item);
-    const StepDesc d = ds[k];
-    const int p = parity ^ (d.flags & 1);
-    const uint32_t tile = item - d.tile_begin;
-    switch (d.cpt) {
-      case 2: run_tile<FORM, 2, 0>(d, p, tile, flags); break;
-      case 4: run_tile<FORM, 4, 0>(d, p, tile, flags); break;
-      default: run_tile<FORM, 1, 0>(d, p, tile, flags); break;
-    }
-    // the next item's slot stage rewrites the LDS this item's flush reads
-    if (flags & (WT_FLAG_FORCES | WT_FLAG_MASS)) __syncthreads();
+  const uint32_t item = backwards ? nitems - 1u - blockIdx.x : blockIdx.x;
+  int lo = 0, hi = nd - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (ds[mid].tile_begin <= item) lo = mid; else hi = mid - 1;
+  }
+  const StepDesc d = ds[lo];
+  const int p = parity ^ (d.flags & 1);
+  const uint32_t tile = item - d.tile_begin;
+  switch (d.cpt) {
+    case 2: run_tile<FORM, 2, 0>(d, p, tile, flags); break;
+    case 4: run_tile<FORM, 4, 0>(d, p, tile, flags); break;
+    default: run_tile<FORM, 1, 0>(d, p, tile, flags); break;
   }
 }
 
@@ -1102,20 +1094,10 @@ void launch_step_adv_f(const StepDesc& d, const AdvectArgs& a, Shape sh, int fla
 }
 
 template <int FORM>
-void launch_batch_f(const StepDesc* dd, int nd, uint32_t nitems, uint32_t blocks, int parity, int backwards,
-                    int flags, hipStream_t s) {
-  hipLaunchKernelGGL((k_step_batch<FORM>), dim3(blocks), dim3(BLOCK), 0, s, dd, nd, nitems, parity, backwards,
+void launch_batch_f(const StepDesc* dd, int nd, uint32_t nitems, int parity, int backwards, int flags,
+                    hipStream_t s) {
+  hipLaunchKernelGGL((k_step_batch<FORM>), dim3(nitems), dim3(BLOCK), 0, s, dd, nd, nitems, parity, backwards,
                      flags);
-}
-// Workgroups of one full occupancy of the batched kernel on the current device.
-template <int FORM>
-int batch_blocks_f() {
-  int dev = 0, ncu = 0, per_cu = 0;
-  if (hipGetDevice(&dev) != hipSuccess) return -1;
-  if (hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess) return -1;
-  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void*)k_step_batch<FORM>, BLOCK, 0) != hipSuccess)
-    return -1;
-  return ncu * std::max(per_cu, 1);
 }
 
 // The FP32 step kernels are instantiated in their own translation unit,
@@ -1127,18 +1109,16 @@ int batch_blocks_f() {
 template void launch_step_f<WT_FP32>(const StepDesc& d, int parity, Shape sh, int flags, hipStream_t s);
 template void launch_step_adv_f<WT_FP32>(const StepDesc& d, const AdvectArgs& a, Shape sh, int flags,
                                         hipStream_t s);
-template void launch_batch_f<WT_FP32>(const StepDesc* dd, int nd, uint32_t nitems, uint32_t blocks, int parity,
-                                      int backwards, int flags, hipStream_t s);
-template int batch_blocks_f<WT_FP32>();
+template void launch_batch_f<WT_FP32>(const StepDesc* dd, int nd, uint32_t nitems, int parity, int backwards,
+                                      int flags, hipStream_t s);
 
 }  // namespace wt
 #else
 extern template void launch_step_f<WT_FP32>(const StepDesc& d, int parity, Shape sh, int flags, hipStream_t s);
 extern template void launch_step_adv_f<WT_FP32>(const StepDesc& d, const AdvectArgs& a, Shape sh, int flags,
                                                hipStream_t s);
-extern template void launch_batch_f<WT_FP32>(const StepDesc* dd, int nd, uint32_t nitems, uint32_t blocks,
-                                             int parity, int backwards, int flags, hipStream_t s);
-extern template int batch_blocks_f<WT_FP32>();
+extern template void launch_batch_f<WT_FP32>(const StepDesc* dd, int nd, uint32_t nitems, int parity,
+                                             int backwards, int flags, hipStream_t s);
 
 static int64_t nslots_of(int64_t n) { return (n + SLOT - 1) / SLOT; }
 static int64_t nchunks_of(int64_t n) { return (nslots_of(n) + CHUNK - 1) / CHUNK; }
@@ -1437,14 +1417,6 @@ int wt_batch_prepare(const wt_grid* grids, const wt_params* params, const int* c
       rs[k] = RedDesc{grids[k].partials, grids[k].sums, grids[k].sums, (uint32_t)nslots_of(d.n)};
     max_chunks = std::max<int64_t>(max_chunks, nchunks_of(d.n));
   }
-  int blocks = -1;
-  switch (form) {
-    case WT_FP32: blocks = batch_blocks_f<WT_FP32>(); break;
-    case WT_FP16: blocks = batch_blocks_f<WT_FP16>(); break;
-    case WT_BF16: blocks = batch_blocks_f<WT_BF16>(); break;
-    case WT_FP64: blocks = batch_blocks_f<WT_FP64>(); break;
-  }
-  if (blocks <= 0) return fail(-2, "occupancy query of the batched step failed: %s", hipGetErrorString(hipGetLastError()));
   hipStream_t s = (hipStream_t)stream;
   char* ws = (char*)workspace;
   if (hipMemcpyAsync(ws, ds.data(), sizeof(StepDesc) * ntunnels, hipMemcpyHostToDevice, s) != hipSuccess ||
@@ -1457,9 +1429,8 @@ int wt_batch_prepare(const wt_grid* grids, const wt_params* params, const int* c
   info->max_chunks = (int32_t)max_chunks;
   info->form = form;
   info->ntunnels = ntunnels;
-  info->blocks = (int32_t)std::min<int64_t>(blocks, (int64_t)tile);
   info->tile_order = 2;
-  info->reserved = 0;
+  info->reserved[0] = info->reserved[1] = 0;
   return 0;
 }
 
@@ -1470,7 +1441,6 @@ int wt_batch_step(void* workspace, const wt_batch_info* info, int parity, int fl
   if (!workspace || !info || info->ntunnels <= 0 || info->total_tiles <= 0) return fail(-1, "empty batch");
   const int form = info->form;
   if (form < WT_FP32 || form > WT_FP64) return fail(-1, "unknown storage form %d", form);
-  if (info->blocks <= 0) return fail(-1, "batch launch needs at least one workgroup (got %d)", info->blocks);
   if (info->tile_order & ~2) return fail(-1, "unknown batch tile order bits 0x%x", info->tile_order);
   hipStream_t s = (hipStream_t)stream;
   const int nt = info->ntunnels;
@@ -1478,13 +1448,12 @@ int wt_batch_step(void* workspace, const wt_batch_info* info, int parity, int fl
   const RedDesc* rd = (const RedDesc*)((const char*)workspace + sizeof(StepDesc) * nt);
   const int f = flags & (WT_FLAG_FORCES | WT_FLAG_MASS);
   const uint32_t nitems = (uint32_t)info->total_tiles;
-  const uint32_t blocks = (uint32_t)std::min<int64_t>(info->blocks, info->total_tiles);
   const int backwards = ((info->tile_order & 2) && (parity & 1)) ? 1 : 0;
   switch (form) {
-    case WT_FP32: launch_batch_f<WT_FP32>(dd, nt, nitems, blocks, parity, backwards, f, s); break;
-    case WT_FP16: launch_batch_f<WT_FP16>(dd, nt, nitems, blocks, parity, backwards, f, s); break;
-    case WT_BF16: launch_batch_f<WT_BF16>(dd, nt, nitems, blocks, parity, backwards, f, s); break;
-    case WT_FP64: launch_batch_f<WT_FP64>(dd, nt, nitems, blocks, parity, backwards, f, s); break;
+    case WT_FP32: launch_batch_f<WT_FP32>(dd, nt, nitems, parity, backwards, f, s); break;
+    case WT_FP16: launch_batch_f<WT_FP16>(dd, nt, nitems, parity, backwards, f, s); break;
+    case WT_BF16: launch_batch_f<WT_BF16>(dd, nt, nitems, parity, backwards, f, s); break;
+    case WT_FP64: launch_batch_f<WT_FP64>(dd, nt, nitems, parity, backwards, f, s); break;
   }
   if (f) {
     hipLaunchKernelGGL(k_reduce1, dim3((unsigned)info->max_chunks, (unsigned)nt), dim3(BLOCK), 0, s, rd);

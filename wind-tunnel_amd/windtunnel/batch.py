@@ -1,6 +1,6 @@
-"""Many resident tunnels advanced together: one persistent launch of the batched
-step kernel per lattice step for all tunnels of one storage form, each tunnel
-keeping the launch shape (cells per thread) of its size class.
+"""Many resident tunnels advanced together: one launch of the batched step kernel
+per lattice step for all tunnels of one storage form, each tunnel keeping the
+launch shape (cells per thread) of its size class.
 
 The descriptor table (per-tunnel buffers, sizes, parameters, tile offsets) is
 uploaded once per batch composition (wt_batch_prepare); each step is then a
@@ -40,7 +40,7 @@ class Batch:
     def prepare(self, stream=None):
         nt = len(self.tunnels)
         # each tunnel's grid carries the launch shape of its size class: its tiles
-        # hold 256 * cpt cells inside the one persistent launch
+        # hold 256 * cpt cells inside the one launch
         grids = (Grid * nt)(*[t.grid for t in self.tunnels])
         prm = (Params * nt)(*[Params(*p) for p in self.params])
         cur = (ctypes.c_int32 * nt)(*[t.cur for t in self.tunnels])

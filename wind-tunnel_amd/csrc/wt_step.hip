@@ -184,6 +184,76 @@ __device__ __forceinline__ uint32_t gather_force_links(const uint8_t* cls, uint3
   return m;
 }
 
+// Link classes staged in LDS. A wave whose cells touch a wall (any lane flagged
+// CLS_NEAR) copies the class bytes of the 9 rows its cells' neighbours lie in --
+// rows (dy, dz) in {-1, 0, 1}^2 around its 64 consecutive cells, one cell beyond
+// each end, from the dword boundary below -- into its own slice of LDS with
+// 2-3 coalesced dword loads per lane, then every lane reads its 18 upstream
+// (lbm.py:114-124) and 18 downstream (lbm.py:185-188) neighbour classes from
+// LDS instead of gathering 36 single bytes from memory. The loads go through a
+// raw buffer of the class array's n bytes: a window reaching before the first
+// or past the last cell reads zeros there (offsets below 0 wrap above n), and
+// only lanes of interior cells, whose neighbours all lie inside, use them.
+// A wave reads back only what it wrote itself, so no workgroup barrier is needed.
+constexpr uint32_t LINK_DW = 18;  // dwords per row window: 66 bytes from a dword boundary
+__device__ __forceinline__ uint32_t row_start(uint32_t idx0, int w, int32_t nx, int32_t nxy) {
+  const int dy = w / 3 - 1, dz = w % 3 - 1;
+  return idx0 + (uint32_t)(dy * nx + dz * nxy) - 1u;
+}
+__device__ __forceinline__ void stage_link_rows(uint32_t (*rows)[LINK_DW], const uint8_t* cls, uint32_t n,
+                                                uint32_t idx0, int32_t nx, int32_t nxy) {
+  const __amdgpu_buffer_rsrc_t r =
+      __builtin_amdgcn_make_buffer_rsrc((void*)cls, (short)0, (int)n, 0x00020000);
+  const uint32_t lane = threadIdx.x & 63;
+  uint32_t* flat = &rows[0][0];            // dword q is row q / LINK_DW, dword q % LINK_DW
+  auto offset = [&](uint32_t q) {
+    const uint32_t w = q / LINK_DW, k = q - w * LINK_DW;
+    return (row_start(idx0, (int)w, nx, nxy) & ~3u) + 4u * k;
+  };
+  // the active lanes are a prefix of the wave (a tunnel's last wave may be partial)
+  const uint32_t m = (uint32_t)__popcll(__ballot(1));
+  if (m == 64) {
+    // a full wave: the 162 dwords as 3 loads per lane in flight together
+    const uint32_t q2 = lane + 128;
+    const uint32_t v0 = __builtin_amdgcn_raw_buffer_load_b32(r, offset(lane), 0, 0);
+    const uint32_t v1 = __builtin_amdgcn_raw_buffer_load_b32(r, offset(lane + 64), 0, 0);
+    const uint32_t v2 = q2 < 9 * LINK_DW ? __builtin_amdgcn_raw_buffer_load_b32(r, offset(q2), 0, 0) : 0u;
+    flat[lane] = v0;
+    flat[lane + 64] = v1;
+    if (q2 < 9 * LINK_DW) flat[q2] = v2;
+  } else {
+    for (uint32_t q = lane; q < 9 * LINK_DW; q += m) flat[q] = __builtin_amdgcn_raw_buffer_load_b32(r, offset(q), 0, 0);
+  }
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+// class code of the neighbour (dx, dy, dz) of the lane's cell, from the staged rows
+__device__ __forceinline__ uint32_t staged_code(const uint32_t (*rows)[LINK_DW], uint32_t idx0, int32_t nx,
+                                                int32_t nxy, int dx, int dy, int dz) {
+  const int w = (dy + 1) * 3 + (dz + 1);
+  const uint32_t p = (row_start(idx0, w, nx, nxy) & 3u) + (threadIdx.x & 63) + 1u + (uint32_t)dx;
+  return reinterpret_cast<const uint8_t*>(rows[w])[p] & CLS_CODE;
+}
+__device__ __forceinline__ Links staged_links(const uint32_t (*rows)[LINK_DW], uint32_t idx0, int32_t nx,
+                                              int32_t nxy) {
+  Links l{0u, 0u};
+#pragma unroll
+  for (int i = 1; i < Q; i++) {
+    const uint32_t t = staged_code(rows, idx0, nx, nxy, -cx(i), -cy(i), -cz(i));
+    l.bb |= (t != 0u ? 1u : 0u) << i;
+    if (cx(i) != 0) l.belt |= (t == 2u ? 1u : 0u) << i;
+  }
+  return l;
+}
+__device__ __forceinline__ uint32_t staged_force_links(const uint32_t (*rows)[LINK_DW], uint32_t idx0, int32_t nx,
+                                                       int32_t nxy) {
+  uint32_t m = 0;
+#pragma unroll
+  for (int i = 1; i < Q; i++) m |= (staged_code(rows, idx0, nx, nxy, cx(i), cy(i), cz(i)) == 1u ? 1u : 0u) << i;
+  return m;
+}
+
 struct Acc {
   double fx, fy, fz, m;
 };
@@ -404,11 +474,9 @@ __device__ __forceinline__ void collide_form(typename Form<FORM>::real (&f)[Q], 
 // Momentum exchange of one interior cell on the populations it stores
 // (lbm.py:185-193): links i whose downstream neighbour is a static solid.
 template <int FORM>
-__device__ __forceinline__ void add_force(const uint8_t* cls, uint32_t idx, int32_t nx, int32_t nxy,
-                                          const typename Form<FORM>::elem (&st)[Q], Acc& acc) {
+__device__ __forceinline__ void add_force(uint32_t fl, const typename Form<FORM>::elem (&st)[Q], Acc& acc) {
   typedef Form<FORM> S;
   typedef typename S::real R;
-  const uint32_t fl = gather_force_links(cls, idx, nx, nxy);
 #pragma unroll
   for (int i = 1; i < Q; i++) {
     if ((fl >> i) & 1u) {
@@ -505,6 +573,11 @@ __device__ __forceinline__ void step_cell_general(const StepDesc& d, const typen
   const int32_t nx = d.nx, ny = d.ny;
   const int32_t nxy = nx * ny;
   const R zero = (R)0, one = (R)1;
+  // class bytes of the wave's neighbour rows, when any lane touches a wall
+  __shared__ uint32_t link_rows[BLOCK / 64][9][LINK_DW];
+  uint32_t (*rows)[LINK_DW] = link_rows[threadIdx.x >> 6];
+  const uint32_t idx0 = idx - (threadIdx.x & 63);
+  if (__any((c & CLS_NEAR) != 0)) stage_link_rows(rows, d.cls, d.n, idx0, nx, nxy);
 
   if ((c & CLS_CODE) != 0) {  // solid cell, lbm.py:81-83
     st_mac<POL == 1>(d.mac + idx, 0.f, 0.f, 0.f, 1.f);
@@ -551,7 +624,7 @@ __device__ __forceinline__ void step_cell_general(const StepDesc& d, const typen
   // from the cell's own opposite plane pc + ppos(opp(i)) * nx when that
   // neighbour is a wall; the loads go out in ascending upstream-row order
   Links lk{0u, 0u};
-  if (c & CLS_NEAR) lk = gather_links(d.cls, idx, nx, nxy);
+  if (c & CLS_NEAR) lk = staged_links(rows, idx0, nx, nxy);
   R f[Q];
   auto global_loads = [&]() {
 #pragma unroll
@@ -599,7 +672,7 @@ __device__ __forceinline__ void step_cell_general(const StepDesc& d, const typen
   st_mac<POL == 1>(d.mac + idx, (float)ux, (float)uy, (float)uz, (float)rho);
   shell_write<R>(shout, x, y, z, ux, uy, uz, rho);
   if (flags & WT_FLAG_MASS) acc.m += (double)rho;
-  if ((flags & WT_FLAG_FORCES) && (c & CLS_FORCE)) add_force<FORM>(d.cls, idx, nx, nxy, st, acc);
+  if ((flags & WT_FLAG_FORCES) && (c & CLS_FORCE)) add_force<FORM>(staged_force_links(rows, idx0, nx, nxy), st, acc);
 }
 
 // ---- one cell, fast path: the whole wave is interior fluid away from walls -----------
@@ -1029,16 +1102,21 @@ __global__ __launch_bounds__(BLOCK) void k_link_classes(const uint8_t* __restric
   const int y = (int)(yz - zz * (uint32_t)ny);
   const int z = (int)zz;
   const uint8_t c = cls[idx];
+  // the step kernel's own path: neighbour classes staged in LDS per wave
+  __shared__ uint32_t link_rows[BLOCK / 64][9][LINK_DW];
+  uint32_t (*rows)[LINK_DW] = link_rows[threadIdx.x >> 6];
+  const uint32_t idx0 = idx - (threadIdx.x & 63);
+  const int32_t nxy = nx * ny;
+  if (__any((c & CLS_NEAR) != 0)) stage_link_rows(rows, cls, n, idx0, nx, nxy);
   uint64_t v;
   if ((c & CLS_CODE) != 0) {
     v = 1;
   } else if ((x == 0) | (x == nx - 1) | (y == 0) | (y == ny - 1) | (z == 0) | (z == nz - 1)) {
     v = 2;
   } else {
-    const int32_t nxy = nx * ny;
     Links lk{0u, 0u};
-    if (c & CLS_NEAR) lk = gather_links(cls, idx, nx, nxy);
-    const uint32_t fl = (c & CLS_FORCE) ? gather_force_links(cls, idx, nx, nxy) : 0u;
+    if (c & CLS_NEAR) lk = staged_links(rows, idx0, nx, nxy);
+    const uint32_t fl = (c & CLS_FORCE) ? staged_force_links(rows, idx0, nx, nxy) : 0u;
     v = ((uint64_t)lk.bb << 8) | ((uint64_t)lk.belt << 27) | ((uint64_t)fl << 45);
   }
   out[idx] = v;
@@ -1055,13 +1133,20 @@ struct Shape {
   int solo;   // the cache policy of the launch (run_tile's POL: 0, 1 or 2)
 };
 
+// Workgroups of the FP32 single-tunnel step kernels reserve LDS they do not use,
+// which holds them to 5 per CU (5 waves per SIMD, 160 KB of LDS per CU): at 80
+// VGPRs the hardware would run 6 waves, and at 33.5M cells 6 waves measured
+// 1.7 % slower per session step than 5 (FP16/BF16 run best at their own 7).
+constexpr size_t FP32_STEP_PAD_LDS = 28 * 1024;
+template <int FORM> constexpr size_t step_pad_lds() { return FORM == WT_FP32 ? FP32_STEP_PAD_LDS : 0; }
+
 template <int FORM, int CPT>
 static void launch_step_t(const StepDesc& d, int parity, int remap, int flags, int solo, hipStream_t s) {
   const uint32_t ntile = (d.n + BLOCK * CPT - 1) / (BLOCK * CPT);
   switch (solo) {
-    case 1: hipLaunchKernelGGL((k_step<FORM, CPT, 1>), dim3(ntile), dim3(BLOCK), 0, s, d, parity, remap, flags); break;
-    case 2: hipLaunchKernelGGL((k_step<FORM, CPT, 2>), dim3(ntile), dim3(BLOCK), 0, s, d, parity, remap, flags); break;
-    default: hipLaunchKernelGGL((k_step<FORM, CPT, 0>), dim3(ntile), dim3(BLOCK), 0, s, d, parity, remap, flags); break;
+    case 1: hipLaunchKernelGGL((k_step<FORM, CPT, 1>), dim3(ntile), dim3(BLOCK), step_pad_lds<FORM>(), s, d, parity, remap, flags); break;
+    case 2: hipLaunchKernelGGL((k_step<FORM, CPT, 2>), dim3(ntile), dim3(BLOCK), step_pad_lds<FORM>(), s, d, parity, remap, flags); break;
+    default: hipLaunchKernelGGL((k_step<FORM, CPT, 0>), dim3(ntile), dim3(BLOCK), step_pad_lds<FORM>(), s, d, parity, remap, flags); break;
   }
 }
 
@@ -1079,9 +1164,9 @@ static void launch_step_adv_t(const StepDesc& d, const AdvectArgs& a, int remap,
   const uint32_t ntile = (d.n + BLOCK * CPT - 1) / (BLOCK * CPT);
   const uint32_t nadv = (a.num_p + BLOCK - 1) / BLOCK;
   switch (solo) {
-    case 1: hipLaunchKernelGGL((k_step_adv<FORM, CPT, 1>), dim3(ntile + nadv), dim3(BLOCK), 0, s, d, a, 0, remap, flags, ntile); break;
-    case 2: hipLaunchKernelGGL((k_step_adv<FORM, CPT, 2>), dim3(ntile + nadv), dim3(BLOCK), 0, s, d, a, 0, remap, flags, ntile); break;
-    default: hipLaunchKernelGGL((k_step_adv<FORM, CPT, 0>), dim3(ntile + nadv), dim3(BLOCK), 0, s, d, a, 0, remap, flags, ntile); break;
+    case 1: hipLaunchKernelGGL((k_step_adv<FORM, CPT, 1>), dim3(ntile + nadv), dim3(BLOCK), step_pad_lds<FORM>(), s, d, a, 0, remap, flags, ntile); break;
+    case 2: hipLaunchKernelGGL((k_step_adv<FORM, CPT, 2>), dim3(ntile + nadv), dim3(BLOCK), step_pad_lds<FORM>(), s, d, a, 0, remap, flags, ntile); break;
+    default: hipLaunchKernelGGL((k_step_adv<FORM, CPT, 0>), dim3(ntile + nadv), dim3(BLOCK), step_pad_lds<FORM>(), s, d, a, 0, remap, flags, ntile); break;
   }
 }
 template <int FORM>

@@ -7,10 +7,10 @@ advects the particles with dt = 1 (advect_kernel, lbm.py:233-263), and on frame
 steps extracts slice planes (slice_kernel, lbm.py:278-318) and queues
 asynchronous copies of slices and forces into pinned host buffers.
 
-With fuse=True the advection of step k runs inside the launch of step k + 1
-(wt_step_advect) instead of its own launch after step k. Results are identical
-either way; by default every form launches the advection separately, which
-measured faster at 33.5M cells and 2^20 particles (FUSED_FORMS below).
+The advection of step k runs inside the launch of step k + 1 (wt_step_advect)
+instead of its own launch after step k (fuse=False keeps separate launches).
+Results are identical either way; the fused launch measured faster at 33.5M
+cells and 2^20 particles (FUSED_FORMS below).
 """
 import numpy as np
 import torch
@@ -20,12 +20,12 @@ from .coefficients import ForceAverage
 from .tunnel import Tunnel
 
 # storage forms whose sessions fuse the particle advection into the next step's
-# launch (Session(fuse=True) asks for it in any form). None by default: with the
-# FP32 step kernels at 88 VGPRs (5 waves per SIMD, which the fused launch's
-# advection blocks then run at too) the fused loop measured 0.05-0.4 % slower per
-# step than a separate advection launch at 33.5M cells and 2^20 particles, and
-# FP16's 0.1-0.6 % slower.
-FUSED_FORMS = ()
+# launch by default (Session(fuse=...) overrides). With the clamp's square root
+# taken only where it can fire and the deviation-form FP16/BF16 collision, the
+# fused loop measured 0.15 % (FP32) and 1.4 % (FP16) faster per step than a
+# separate advection launch at 33.5M cells and 2^20 particles, three
+# alternating same-process repeats (0.9765 vs 0.9780 ms, 0.5796 vs 0.5880 ms).
+FUSED_FORMS = ('fp32', 'fp16', 'bf16', 'fp64')
 # frames whose forces the device-side log holds between two reads of the average
 FRAME_LOG = 4096
 

@@ -122,6 +122,15 @@ int wt_set_solid(const wt_grid* g, const uint8_t* solid_dev, void* stream);
  * in g->sums when the stream reaches the end of this call. Replaces the step
  * loop of Solver.step (lbm.py:442-447) over step_kernel (lbm.py:72-166). */
 int wt_step(const wt_grid* g, int cur, int nsteps, const wt_params* p, int flags, void* stream);
+/* wt_step without allocation of any kind, for callers that cannot allocate
+ * device memory (code under HIP graph capture, tight serving loops, harnesses
+ * that forbid hipMalloc). No counterpart in the reference. The force/mass
+ * reduction slots live in a static device array of the library (8.4 MB per
+ * device, held until the process ends) instead of g->partials, which may be
+ * NULL; g->sums receives the sums as with wt_step. Tunnels of at most 2^26
+ * cells (larger ones are rejected with an error). Calls that overlap on one
+ * device share that array and race: serialize them (one stream, or events). */
+int wt_step_noalloc(const wt_grid* g, int cur, int nsteps, const wt_params* p, int flags, void* stream);
 /* One lattice step as wt_step(g, cur, 1, p, flags, stream), fused in the same
  * launch with the particle advection of the previous step (slot storage, as
  * wt_advect_slots; ids may be NULL when slot = id) sampled from mac_prev, the

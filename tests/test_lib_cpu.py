@@ -65,6 +65,18 @@ def test_launch_shape_is_checked_per_grid_before_any_launch():
     del fake
 
 
+def test_noalloc_step_rejects_tunnels_beyond_its_scratch():
+    """wt_step_noalloc serves tunnels of at most 2^26 cells and says so, before any
+    device call (the fake buffer addresses are never dereferenced)."""
+    from windtunnel import _lib
+    lib = _lib.load()
+    g = _lib.Grid(512, 512, 257, 0, 1, 0, (ctypes.c_void_p * 2)(0x1000, 0x2000), 0x3000, 0x4000,
+                  (ctypes.c_void_p * 2)(0x5000, 0x6000), 0, 0x8000)
+    p = _lib.Params(0.05, 0.52, 0.16)
+    assert lib.wt_step_noalloc(ctypes.byref(g), 0, 1, ctypes.byref(p), 1, None) < 0
+    assert 'exceed' in _lib.last_error() and str(1 << 26) in _lib.last_error()
+
+
 @pytest.mark.parametrize('dims,form,policy', [
     ((128, 64, 64), 0, 2),      # 2 x 19 x 4 B x 0.52M = 80 MB: all cached
     ((160, 96, 96), 0, 2),      # 213.75 MiB, within the 224 MiB bound

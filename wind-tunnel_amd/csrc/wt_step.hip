@@ -36,6 +36,7 @@
 
 #include <algorithm>
 #include <cstdio>
+#include <mutex>
 #include <vector>
 
 #include "wt_device.h"
@@ -1246,6 +1247,38 @@ int check_grid(const wt_grid* g, bool need_state) {
   return 0;
 }
 
+// ---- scratch of the allocation-free step ------------------------------------------
+// wt_step_noalloc keeps the FP64 reduction slots of the force/mass epilogue
+// (4 doubles per 256-cell slot, then 4 per 1024-slot chunk) in this array
+// instead of the grid's `partials` buffer: 8.4 MB on every device the library
+// runs on, held until the process ends, enough for tunnels of up to 2^26 cells.
+// Every slot and chunk sum a call reads is written earlier in the same call, so
+// nothing is cleared; calls that overlap on one device share it and race.
+constexpr int64_t NOALLOC_MAX_CELLS = (int64_t)1 << 26;
+constexpr int64_t NOALLOC_SLOTS = NOALLOC_MAX_CELLS / SLOT;
+constexpr int64_t NOALLOC_CHUNKS = NOALLOC_SLOTS / CHUNK;
+static __device__ __align__(256) double scratch[4 * (NOALLOC_SLOTS + NOALLOC_CHUNKS)];
+
+// Device address of `scratch` on the current device, looked up once per device
+// with that device current (each device holds its own copy at its own address).
+// Host code uses this pointer, never the array's name, for the array's memory.
+static int scratch_ptr(double** out) {
+  static std::mutex lock;
+  static double* ptrs[64] = {nullptr};
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess) return fail(-2, "hipGetDevice failed: %s", hipGetErrorString(hipGetLastError()));
+  if (dev < 0 || dev >= 64) return fail(-1, "device %d beyond the 64 this library tracks", dev);
+  std::lock_guard<std::mutex> g(lock);
+  if (!ptrs[dev]) {
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p) return fail(-2, "hipGetSymbolAddress of the step scratch failed: %s", hipGetErrorString(e));
+    ptrs[dev] = (double*)p;
+  }
+  *out = ptrs[dev];
+  return 0;
+}
+
 static int reduce_one(const wt_grid* g, hipStream_t s) {
   const int64_t n = (int64_t)g->nx * g->ny * g->nz;
   if (!g->partials || !g->sums) return fail(-1, "grid has no reduction buffers");
@@ -1389,6 +1422,19 @@ int wt_step(const wt_grid* g, int cur, int nsteps, const wt_params* p, int flags
     return reduce_one(g, s);
   }
   return check_launch("wt_step");
+}
+
+int wt_step_noalloc(const wt_grid* g, int cur, int nsteps, const wt_params* p, int flags, void* stream) {
+  if (int e = check_grid(g, false)) return e;
+  const int64_t n = (int64_t)g->nx * g->ny * g->nz;
+  if (n > NOALLOC_MAX_CELLS)
+    return fail(-1, "wt_step_noalloc: %lld cells exceed its scratch's maximum of %lld; use wt_step",
+                (long long)n, (long long)NOALLOC_MAX_CELLS);
+  double* sp = nullptr;
+  if (int e = scratch_ptr(&sp)) return e;
+  wt_grid h = *g;
+  h.partials = sp;       // the reduction slots live in the static scratch
+  return wt_step(&h, cur, nsteps, p, flags, stream);
 }
 
 int wt_step_advect(const wt_grid* g, int cur, const wt_params* p, int flags, const float* mac_prev, float* pos,

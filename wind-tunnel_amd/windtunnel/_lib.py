@@ -59,6 +59,7 @@ SIGNATURES = {
     'wt_init': (c_i32, [P(Grid), c_f32, c_vp]),
     'wt_set_solid': (c_i32, [P(Grid), c_vp, c_vp]),
     'wt_step': (c_i32, [P(Grid), c_i32, c_i32, P(Params), c_i32, c_vp]),
+    'wt_step_noalloc': (c_i32, [P(Grid), c_i32, c_i32, P(Params), c_i32, c_vp]),
     'wt_step_advect': (c_i32, [P(Grid), c_i32, P(Params), c_i32, c_vp, c_vp, c_vp, c_vp, c_i64, c_f32, c_f32, c_f32,
                                c_u32, P(SeedBox), c_vp]),
     'wt_forces': (c_i32, [P(Grid), c_i32, c_vp]),

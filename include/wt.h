@@ -173,6 +173,14 @@ int64_t wt_batch_workspace_bytes(int ntunnels);
  * fetches the whole batch; otherwise in grids[k].sums. Synchronizes `stream`. */
 int wt_batch_prepare(const wt_grid* grids, const wt_params* params, const int* cur, int ntunnels,
                      double* sums_dev, void* workspace, wt_batch_info* info, void* stream);
+/* The descriptor table wt_batch_prepare uploads, written into host memory
+ * (wt_batch_workspace_bytes(ntunnels) bytes at host_image) with info filled
+ * in; no device call. A caller that copies host_image to the workspace in
+ * stream order (e.g. from pinned memory) can change a batch's membership
+ * without synchronizing: the serving loop does, so that admitting or retiring
+ * a session never drains the ticks in flight. No counterpart in the reference. */
+int wt_batch_encode(const wt_grid* grids, const wt_params* params, const int* cur, int ntunnels,
+                    double* sums_dev, void* host_image, wt_batch_info* info);
 /* One step of every tunnel of the prepared batch; with parity p tunnel k
  * reads buffer cur[k]^p. flags as for wt_step; sums land in each grid's sums.
  * Kernel launches only: a caller may capture it in a graph. */

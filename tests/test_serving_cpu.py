@@ -47,3 +47,20 @@ def test_tick_budget_learns_median_rate():
     tb.observe(0.01, 0.0)
     assert len(tb.samples) == 5
     assert tb.predicted(2e7) == 2e7 * tb.spu
+
+
+def test_prefilled_workload():
+    """prefill=N adds N sessions resident at t = 0 (steps still to run drawn over
+    a length-biased session length) in front of the unchanged-in-kind Poisson
+    stream; the plain stream (no prefill) is the recorded one."""
+    import numpy as np
+    from windtunnel.serving import mixed_workload
+    plain = mixed_workload(seed=0, duration=600.0, rate=2.0)
+    assert len(plain) == 1183 and plain[0].dims == (176, 88, 88) and plain[0].steps == 396
+    reqs = mixed_workload(seed=0, duration=30.0, rate=2.0, prefill=40)
+    assert all(r.arrival == 0.0 for r in reqs[:40]) and all(r.arrival > 0.0 for r in reqs[40:])
+    assert all(1 <= r.steps <= 5000 for r in reqs[:40]) and all(200 <= r.steps <= 5000 for r in reqs[40:])
+    assert [r.sid for r in reqs] == list(range(len(reqs)))
+    cells = np.array([np.prod(r.dims) for r in reqs])
+    assert cells.min() >= 0.4e6 and cells.max() <= 34e6
+    assert len({r.form for r in reqs[:40]}) == 2

@@ -1515,14 +1515,14 @@ int64_t wt_batch_workspace_bytes(int ntunnels) {
   return (int64_t)ntunnels * (int64_t)(sizeof(StepDesc) + sizeof(RedDesc));
 }
 
-int wt_batch_prepare(const wt_grid* grids, const wt_params* params, const int* cur, int ntunnels,
-                     double* sums_dev, void* workspace, wt_batch_info* info, void* stream) {
-  if (!grids || !params || !cur || !workspace || !info || ntunnels <= 0)
+int wt_batch_encode(const wt_grid* grids, const wt_params* params, const int* cur, int ntunnels,
+                    double* sums_dev, void* host_image, wt_batch_info* info) {
+  if (!grids || !params || !cur || !host_image || !info || ntunnels <= 0)
     return fail(-1, "null argument or empty batch");
   const int form = grids[0].form;
   if (form < WT_FP32 || form > WT_FP64) return fail(-1, "unknown storage form %d", form);
-  std::vector<StepDesc> ds(ntunnels);
-  std::vector<RedDesc> rs(ntunnels);
+  StepDesc* ds = (StepDesc*)host_image;
+  RedDesc* rs = (RedDesc*)((char*)host_image + sizeof(StepDesc) * ntunnels);
   uint64_t tile = 0;
   int64_t max_chunks = 0;
   for (int k = 0; k < ntunnels; k++) {
@@ -1548,20 +1548,25 @@ int wt_batch_prepare(const wt_grid* grids, const wt_params* params, const int* c
       rs[k] = RedDesc{grids[k].partials, grids[k].sums, grids[k].sums, (uint32_t)nslots_of(d.n)};
     max_chunks = std::max<int64_t>(max_chunks, nchunks_of(d.n));
   }
-  hipStream_t s = (hipStream_t)stream;
-  char* ws = (char*)workspace;
-  if (hipMemcpyAsync(ws, ds.data(), sizeof(StepDesc) * ntunnels, hipMemcpyHostToDevice, s) != hipSuccess ||
-      hipMemcpyAsync(ws + sizeof(StepDesc) * ntunnels, rs.data(), sizeof(RedDesc) * ntunnels,
-                     hipMemcpyHostToDevice, s) != hipSuccess)
-    return fail(-2, "descriptor upload failed: %s", hipGetErrorString(hipGetLastError()));
-  if (hipStreamSynchronize(s) != hipSuccess)
-    return fail(-2, "stream synchronize failed: %s", hipGetErrorString(hipGetLastError()));
   info->total_tiles = (int64_t)tile;
   info->max_chunks = (int32_t)max_chunks;
   info->form = form;
   info->ntunnels = ntunnels;
   info->tile_order = 2;
   info->reserved[0] = info->reserved[1] = 0;
+  return 0;
+}
+
+int wt_batch_prepare(const wt_grid* grids, const wt_params* params, const int* cur, int ntunnels,
+                     double* sums_dev, void* workspace, wt_batch_info* info, void* stream) {
+  if (!workspace || ntunnels <= 0) return fail(-1, "null workspace or empty batch");
+  std::vector<char> image((size_t)wt_batch_workspace_bytes(ntunnels));
+  if (int e = wt_batch_encode(grids, params, cur, ntunnels, sums_dev, image.data(), info)) return e;
+  hipStream_t s = (hipStream_t)stream;
+  if (hipMemcpyAsync(workspace, image.data(), image.size(), hipMemcpyHostToDevice, s) != hipSuccess)
+    return fail(-2, "descriptor upload failed: %s", hipGetErrorString(hipGetLastError()));
+  if (hipStreamSynchronize(s) != hipSuccess)
+    return fail(-2, "stream synchronize failed: %s", hipGetErrorString(hipGetLastError()));
   return 0;
 }
 

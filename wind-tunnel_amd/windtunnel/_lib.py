@@ -66,6 +66,7 @@ SIGNATURES = {
     'wt_read_sums': (c_i32, [P(Grid), P(ctypes.c_double), c_vp]),
     'wt_batch_workspace_bytes': (c_i64, [c_i32]),
     'wt_batch_prepare': (c_i32, [P(Grid), P(Params), P(c_i32), c_i32, c_vp, c_vp, P(BatchInfo), c_vp]),
+    'wt_batch_encode': (c_i32, [P(Grid), P(Params), P(c_i32), c_i32, c_vp, c_vp, P(BatchInfo)]),
     'wt_batch_step': (c_i32, [c_vp, P(BatchInfo), c_i32, c_i32, c_vp]),
     'wt_advect': (c_i32, [P(Grid), c_vp, c_vp, c_i64, c_f32, c_f32, c_f32, c_u32, P(SeedBox), c_vp]),
     'wt_advect_ordered': (c_i32, [P(Grid), c_vp, c_vp, c_vp, c_i64, c_f32, c_f32, c_f32, c_u32, P(SeedBox), c_vp]),

@@ -3,7 +3,8 @@ per lattice step for all tunnels of one storage form, each tunnel keeping the
 launch shape (cells per thread) of its size class.
 
 The descriptor table (per-tunnel buffers, sizes, parameters, tile offsets) is
-uploaded once per batch composition (wt_batch_prepare); each step is then a
+encoded on the host once per batch composition (wt_batch_encode) and copied to
+the device in stream order, with no synchronization; each step is then a
 kernel launch only (wt_batch_step), so a step of the whole batch can be
 captured in a HIP graph. Graph replays advance two steps (parity 0 then 1),
 returning every tunnel to the same buffer parity.
@@ -44,9 +45,16 @@ class Batch:
         grids = (Grid * nt)(*[t.grid for t in self.tunnels])
         prm = (Params * nt)(*[Params(*p) for p in self.params])
         cur = (ctypes.c_int32 * nt)(*[t.cur for t in self.tunnels])
-        check(self.lib.wt_batch_prepare(grids, prm, cur, nt, ctypes.c_void_p(self.sums.data_ptr()),
-                                        ctypes.c_void_p(self.workspace.data_ptr()),
-                                        ctypes.byref(self.info), stream_handle(stream)), 'wt_batch_prepare')
+        # the descriptor table is written into pinned host memory and copied in stream
+        # order: no synchronization, so a change of membership never drains the work in
+        # flight (the pinned block returns to torch's host allocator only after the copy)
+        image = torch.empty(self.workspace.numel(), dtype=torch.uint8).pin_memory()
+        check(self.lib.wt_batch_encode(grids, prm, cur, nt, ctypes.c_void_p(self.sums.data_ptr()),
+                                       ctypes.c_void_p(image.data_ptr()), ctypes.byref(self.info)),
+              'wt_batch_encode')
+        s = stream if stream is not None else torch.cuda.current_stream(self.workspace.device)
+        with torch.cuda.stream(s):
+            self.workspace.copy_(image, non_blocking=True)
         self.parity = 0
         self.graph = None
 

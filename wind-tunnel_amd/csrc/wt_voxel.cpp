@@ -20,7 +20,7 @@
 #include <vector>
 
 #include "wt.h"
-#include "wt_host.h"
+#include "wt_error.h"
 
 namespace wt {
 

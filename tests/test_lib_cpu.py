@@ -203,3 +203,25 @@ def test_closure_fills_enclosed_pocket():
             ells.append(c + a)
     mask, counts = geometry.ellipsoids(ells, dims)
     assert mask[15 + 30 * (15 + 30 * 15)] == 1     # centre was fluid, enclosed -> solid
+
+
+@pytest.mark.parametrize('form', ['fp32', 'fp16', 'bf16', 'fp64'])
+def test_autotune_classes_follow_the_library_cache_policy(form):
+    """The tuner's size classes are cut where the library's cache policy changes:
+    its Python rule equals wt_cache_policy on both sides of both boundaries."""
+    from windtunnel import _lib, autotune
+    lib = _lib.load()
+    fid = _lib.FORMS[form]
+    eb = autotune.ELEM_BYTES[form]
+    for bound in (224 << 20, 1 << 31):
+        n0 = bound // (2 * 19 * eb)              # the largest cell count at or below the bound
+        a0 = n0 // 9                             # grids a x 3 x 3 on both sides of it
+        pols = []
+        for a in (a0 - 1, a0, a0 + 1, a0 + 2):
+            pols.append(autotune.cache_policy(form, 9 * a))
+            assert lib.wt_cache_policy(a, 3, 3, fid) == pols[-1], (form, a)
+        assert pols[0] == pols[1] != pols[2] == pols[3], (form, bound, pols)
+    for name, dims in autotune.CLASSES:
+        n = dims[0] * dims[1] * dims[2]
+        assert autotune.size_class(n, form) == name
+        assert lib.wt_cache_policy(*dims, fid) == autotune.cache_policy(form, n)

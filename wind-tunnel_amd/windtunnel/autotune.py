@@ -4,9 +4,10 @@ Shapes: cells per thread (1, 2 or 4 consecutive 256-cell slots per
 workgroup) x tile order (bit 0: XCD-contiguous; bit 1: sweep direction
 alternating every step). Results do not depend on the
 shape (the force/mass slots are reduced identically), only the speed does.
-For each storage form and size class the tuner times every shape on a
+For each storage form and size class (classes follow the cache policy the
+library uses for the tunnel's size) the tuner times every shape on a
 representative tunnel and stores the fastest in tuning.json beside this file;
-tunnels look their shape up by size class at launch time.
+tunnels look their shape up by size class when they are created.
 
     python -m windtunnel.autotune [--forms fp32,fp16,bf16,fp64] [--steps 40]
 """
@@ -18,24 +19,33 @@ import sys
 HERE = os.path.dirname(os.path.abspath(__file__))
 TUNING_PATH = os.environ.get('WT_TUNING_PATH') or os.path.join(HERE, 'tuning.json')
 
-# size classes by cell count, with the tunnel each is tuned on
-CLASSES = [
-    ('S', 0, 200_000, (64, 32, 32)),
-    ('M', 200_000, 2_000_000, (128, 64, 64)),
-    ('L', 2_000_000, 12_000_000, (256, 128, 128)),
-    ('XL', 12_000_000, 1 << 31, (512, 256, 256)),
-]
+# Size classes follow the cache policy the library picks for a tunnel (the bytes
+# of its two population buffers, wt_cache_policy), so that no class straddles a
+# policy boundary: S and M are all-cached (policy 2; S below 200k cells), L the
+# middle policy 0, XL the streaming policy 1. The tunnel each is tuned on:
+CLASSES = [('S', (64, 32, 32)), ('M', (128, 64, 64)), ('L', (256, 128, 128)), ('XL', (512, 256, 256))]
+SMALL_CELLS = 200_000
+ELEM_BYTES = {'fp32': 4, 'fp16': 2, 'bf16': 2, 'fp64': 8}
 SHAPES = [(1, 0), (1, 1), (1, 2), (1, 3), (2, 0), (2, 1), (2, 2), (4, 0), (4, 1)]
 DEFAULT = (1, 0)
 
 _table = None
 
 
-def size_class(cells):
-    for name, lo, hi, _ in CLASSES:
-        if lo <= cells < hi:
-            return name
-    return CLASSES[-1][0]
+def cache_policy(form_name, cells):
+    """The library's single-tunnel cache policy for this many cells (the rule of
+    wt_cache_policy: 1 from 2 GiB of populations, 2 within 224 MiB, else 0)."""
+    b = 2 * 19 * int(cells) * ELEM_BYTES[form_name]
+    return 1 if b >= 1 << 31 else (2 if b <= 224 << 20 else 0)
+
+
+def size_class(cells, form_name='fp32'):
+    pol = cache_policy(form_name, cells)
+    if pol == 1:
+        return 'XL'
+    if pol == 0:
+        return 'L'
+    return 'S' if cells < SMALL_CELLS else 'M'
 
 
 def load():
@@ -51,7 +61,7 @@ def load():
 
 def shape_for(form_name, cells):
     t = load().get(form_name, {})
-    v = t.get(size_class(cells))
+    v = t.get(size_class(cells, form_name))
     return tuple(v) if v else DEFAULT
 
 
@@ -64,7 +74,8 @@ def tune(forms, steps=40):
     for form in forms:
         out['shapes'][form] = {}
         out['timings'][form] = {}
-        for name, _, _, dims in CLASSES:
+        for name, dims in CLASSES:
+            assert size_class(dims[0] * dims[1] * dims[2], form) == name, (form, name, dims)
             ells = geometry.car_body(dims, 7, count=8,
                                      box=((0.2 * dims[0], 0.5 * dims[0]), (1, 0.3 * dims[1]),
                                           (0.35 * dims[2], 0.65 * dims[2])),

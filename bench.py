@@ -17,8 +17,9 @@ steps 6000-10000 of the developed 128x64x64 sphere flow.
 value = MLUPS with FP32 populations (all nx*ny*nz cells per step, as
 server/app.py:180 counts them); the same workload with FP16-deviation storage
 is reported beside it. roofline: algorithmic bytes of the step kernel
-(169 B/cell FP32, 93 B/cell FP16: 19 populations read and written, the class
-byte, the 16 B macroscopic store) over its mean launch time, from HIP events on
+(169 B/cell FP32: 19 populations read and written, the class byte, the 16 B
+float4 macroscopic store; 85 B/cell FP16: 2-byte deviations and an 8 B binary16
+macroscopic store) over its mean launch time, from HIP events on
 the stream the kernel runs on, bracketing every 5th launch of the timed
 region (session steps 0 mod 5: force-free; a pair of events on every step slows
 the loop by ~0.7 %). Each step's particles are advected inside the next
@@ -41,7 +42,7 @@ sys.path.insert(0, os.path.join(REPO, 'wind-tunnel_amd'))
 sys.path.insert(0, REPO)
 
 PEAK_HBM_GBS = 8000.0
-BYTES_PER_CELL = {'fp32': 169, 'fp16': 93, 'bf16': 93, 'fp64': 321}
+BYTES_PER_CELL = {'fp32': 169, 'fp16': 85, 'bf16': 85, 'fp64': 321}
 BYTES_PER_PARTICLE = 40
 EVENT_EVERY, EVENT_PHASE = 5, 0      # HIP events around steps numbered 0 mod 5 (force-free)
 FORM_IDS = {'fp32': 0, 'fp16': 1, 'bf16': 2, 'fp64': 3}

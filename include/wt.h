@@ -42,7 +42,12 @@ extern "C" {
  *             (planes read by the same neighbouring row adjacent).
  *             19*n elements.
  * cls:        per-cell class byte built by wt_set_solid from the solid mask.
- * mac:        float4 (ux, uy, uz, rho) per cell (lbm.py:406).
+ * mac:        macroscopic field per cell (lbm.py:406): FP32 and FP64 forms
+ *             float4 (ux, uy, uz, rho), 16 B; FP16 and BF16 forms four
+ *             binary16 (ux, uy, uz, rho - 1), 8 B, each rounded to nearest
+ *             even from the float32 value (rho - 1 formed in float32). The
+ *             particle, slice and Cp kernels widen the binary16 values to
+ *             float32 and take rho = 1 + (rho - 1) in float32.
  * shell[0,1]: ping-pong snapshot of the macroscopic field on the six layers
  *             next to the domain faces; face cells read the previous step's
  *             values from it.
@@ -64,7 +69,7 @@ typedef struct wt_grid {
   int32_t tile_order;
   void* f[2];
   uint8_t* cls;
-  float* mac;
+  void* mac;
   void* shell[2];
   double* partials;
   double* sums;
@@ -138,7 +143,7 @@ int wt_step_noalloc(const wt_grid* g, int cur, int nsteps, const wt_params* p, i
  * step overwrites: callers alternate two mac buffers. Removes one kernel
  * boundary per step (lbm.py:442-453 launches the step and advect_kernel
  * separately); every result equals the two separate launches bit for bit. */
-int wt_step_advect(const wt_grid* g, int cur, const wt_params* p, int flags, const float* mac_prev, float* pos,
+int wt_step_advect(const wt_grid* g, int cur, const wt_params* p, int flags, const void* mac_prev, float* pos,
                    float* spd, const uint32_t* ids, int64_t num_p, float dt, float uin, float max_age,
                    uint32_t seed, const wt_seed_box* box, void* stream);
 /* Momentum-exchange force on f[cur] into g->sums[0..2] (FP64, deterministic).

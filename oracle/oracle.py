@@ -113,6 +113,23 @@ class OracleTunnel:
             self.cur ^= 1
         self.mass = m.value
 
+    def device_mac(self):
+        """The macroscopic field as the device stores it: float32 (ux, uy, uz, rho)
+        for the FP32/FP64 forms; for FP16/BF16 float16 (ux, uy, uz, rho - 1), each
+        rounded to nearest even from float32, rho - 1 formed in float32."""
+        m = self.mac.astype(np.float32)
+        if self.form in (FP16, BF16):
+            q = m.copy()
+            q[:, 3] = m[:, 3] - np.float32(1.0)
+            return q.astype(np.float16)
+        return m
+
+    def field(self):
+        """(ux, uy, uz, rho) float32 as the device's particle, slice and Cp kernels
+        read it (for FP16/BF16: the float16 values widened, rho = 1 + (rho - 1) in
+        float32)."""
+        return device_field(self.device_mac())
+
     def forces(self):
         out = (ctypes.c_double * 3)()
         lib().oracle_force(self.form, _p(self.f[self.cur]), _p(self.solid), self.nx, self.ny, self.nz, out)
@@ -132,6 +149,17 @@ class OracleTunnel:
         return (W.astype(np.float32)[:, None] + g).astype(dtype)
 
 
+def device_field(mac):
+    """float32 (ux, uy, uz, rho) from a device-layout macroscopic field (float32, or
+    float16 with rho - 1); advect, slice_ and cp take either layout through it."""
+    mac = np.asarray(mac).reshape(-1, 4)
+    if mac.dtype == np.float16:
+        f = mac.astype(np.float32)
+        f[:, 3] = np.float32(1.0) + f[:, 3]
+        return f
+    return mac.astype(np.float32)
+
+
 def link_classes(solid, dims):
     nx, ny, nz = dims
     solid = np.ascontiguousarray(solid, np.uint8)
@@ -143,7 +171,7 @@ def link_classes(solid, dims):
 def advect(pos, spd, mac, solid, dims, dt, uin, max_age, seed, box):
     pos = np.array(pos, np.float32, copy=True).reshape(-1, 4)
     spd = np.array(spd, np.float32, copy=True).reshape(-1)
-    mac = np.ascontiguousarray(mac, np.float32)
+    mac = np.ascontiguousarray(device_field(mac))
     solid = np.ascontiguousarray(solid, np.uint8)
     lib().oracle_advect(_p(pos), _p(spd), _p(mac), _p(solid), pos.shape[0], *dims, float(dt), float(uin),
                         float(max_age), int(seed) & 0xffffffff, *(float(b) for b in box))
@@ -167,7 +195,7 @@ def slice_dims(dims, axis):
 def slice_(mac, solid, dims, axis, index, field, uin, lref, vmin, vmax):
     w, h = slice_dims(dims, axis)
     out = np.zeros(2 * w * h, np.uint8)
-    lib().oracle_slice(_p(np.ascontiguousarray(mac, np.float32)), _p(np.ascontiguousarray(solid, np.uint8)),
+    lib().oracle_slice(_p(np.ascontiguousarray(device_field(mac))), _p(np.ascontiguousarray(solid, np.uint8)),
                        _p(out), *dims, axis, index, field, float(uin), float(lref), float(vmin), float(vmax))
     return out
 
@@ -176,7 +204,7 @@ def cp(mac, solid, verts, norms, dims, uin, cp_range):
     verts = np.ascontiguousarray(verts, np.float32).reshape(-1, 3)
     norms = np.ascontiguousarray(norms, np.float32).reshape(-1, 3)
     out = np.zeros(verts.shape[0], np.int8)
-    lib().oracle_cp(_p(np.ascontiguousarray(mac, np.float32)), _p(np.ascontiguousarray(solid, np.uint8)),
+    lib().oracle_cp(_p(np.ascontiguousarray(device_field(mac))), _p(np.ascontiguousarray(solid, np.uint8)),
                     _p(verts), _p(norms), _p(out), verts.shape[0], *dims, float(uin), float(cp_range))
     return out
 

@@ -52,8 +52,8 @@ def test_two_steps_bit_exact_with_oracle(gpu, body, form, oform):
     raw = o.raw()
     assert f.dtype.itemsize == raw.dtype.itemsize
     assert np.array_equal(f.view(np.uint8), raw.view(np.uint8))
-    omac = o.mac.astype(np.float32)                  # the FP64 form stores its macroscopic field as float32
-    assert np.array_equal(mac.reshape(-1).view(np.uint32), omac.reshape(-1).view(np.uint32))
+    omac = o.device_mac()     # float32 (FP32, FP64) or float16 with rho - 1 (FP16, BF16)
+    assert mac.dtype == omac.dtype and np.array_equal(mac.reshape(-1).view(np.uint8), omac.reshape(-1).view(np.uint8))
     assert np.allclose(sums[:3], o.forces()[:3], rtol=1e-12, atol=0)
 
 

@@ -59,7 +59,7 @@ def test_config3_two_steps_bit_exact_at_full_size(gpu, c3_sphere, form, oform):
     o.set_solid(mask)
     o.step(2, C3_UIN, C3_TAU, C3_SMAG)
     assert _same(f, o.raw()), form
-    assert _same(mac, o.mac.astype(np.float32)), form
+    assert _same(mac, o.device_mac()), form
     np.testing.assert_allclose(sums[:3], o.forces(), rtol=1e-12, atol=1e-12 * np.abs(o.forces()).max())
     np.testing.assert_allclose(sums[3], o.mass, rtol=1e-12)
 
@@ -81,7 +81,7 @@ def test_config3_field_error_against_fp64_at_step_200(gpu, c3_sphere):
         t.init(C3_UIN)
         t.set_solid(mask)
         t.step(200, C3_UIN, C3_TAU, C3_SMAG)
-        m = t.mac[fluid].double()
+        m = t.macroscopic()[fluid].double()
         fields[form] = (m[:, :3].clone(), (m[:, 3] - 1.0) / (1.5 * C3_UIN * C3_UIN))
         del t, m
         torch.cuda.empty_cache()
@@ -121,7 +121,7 @@ def _c4_compare(form, oform, shapes=None, steps=3):
         o.set_solid(geometry.ellipsoids(s['ells'], s['dims'])[0])
         o.step(steps, s['uin'], s['tau'], s['smag'])
         assert _same(t.raw_populations().cpu().numpy(), o.raw()), (form, k, s['dims'])
-        assert _same(t.mac.cpu().numpy(), o.mac.astype(np.float32)), (form, k)
+        assert _same(t.mac.cpu().numpy(), o.device_mac()), (form, k)
         np.testing.assert_allclose(sums[k, :3], o.forces(), rtol=1e-12, atol=1e-12 * np.abs(o.forces()).max())
         np.testing.assert_allclose(sums[k, 3], o.mass, rtol=1e-12)
 

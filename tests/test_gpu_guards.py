@@ -45,7 +45,7 @@ def _run(form, oform, dims, mask, uin, tau, smag, steps, raw=None):
     hits = O.clamp_hits()
     torch.cuda.synchronize()
     assert _same(t.raw_populations().cpu().numpy(), o.raw()), form
-    assert _same(t.mac.cpu().numpy(), o.mac.astype(np.float32)), form
+    assert _same(t.mac.cpu().numpy(), o.device_mac()), form
     sums = t.sums.cpu().numpy()
     np.testing.assert_allclose(sums[:3], o.forces(), rtol=1e-12, atol=1e-12 * np.abs(o.forces()).max())
     np.testing.assert_allclose(sums[3], o.mass, rtol=1e-12)

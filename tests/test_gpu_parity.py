@@ -88,7 +88,7 @@ def test_step_bit_exact_with_forces_and_mass(gpu, name, form):
         assert t.cur == o.cur
         assert _raw_equal(t.raw_populations().cpu().numpy(), o.raw()), (name, chunk)
         mac = t.mac.cpu().numpy()
-        assert _raw_equal(mac, o.mac.astype(np.float32)), (name, chunk)
+        assert _raw_equal(mac, o.device_mac()), (name, chunk)
         sums = t.sums.cpu().numpy()
         np.testing.assert_allclose(sums[:3], o.forces(), rtol=1e-12, atol=1e-15)
         np.testing.assert_allclose(sums[3], o.mass, rtol=1e-12)
@@ -110,7 +110,7 @@ def test_step_bit_exact_ragged_sizes(gpu, dims, name, form):
     o.step(6, uin, tau, smag)
     torch.cuda.synchronize()
     assert _raw_equal(t.raw_populations().cpu().numpy(), o.raw()), dims
-    assert _raw_equal(t.mac.cpu().numpy(), o.mac.astype(np.float32)), dims
+    assert _raw_equal(t.mac.cpu().numpy(), o.device_mac()), dims
     sums = t.sums.cpu().numpy()
     np.testing.assert_allclose(sums[:3], o.forces(), rtol=1e-12, atol=1e-15)
     np.testing.assert_allclose(sums[3], o.mass, rtol=1e-12)
@@ -231,7 +231,7 @@ def test_mid_size_tunnel_bit_exact_with_oracle(gpu, name, form, dims):
     o = _oracle(dims, form, mask, uin)
     o.step(3, uin, tau, smag)
     assert _raw_equal(f, o.raw()), dims
-    assert _raw_equal(mac, o.mac.astype(np.float32)), dims
+    assert _raw_equal(mac, o.device_mac()), dims
     np.testing.assert_allclose(sums[:3], o.forces(), rtol=1e-12, atol=1e-15)
     np.testing.assert_allclose(sums[3], o.mass, rtol=1e-12)
 
@@ -277,15 +277,18 @@ def test_repeatable(gpu):
 
 
 # ---- particles, slices, surface pressure -------------------------------------------------------
-def _flow_state(dims=(60, 24, 22), steps=25):
+def _flow_state(dims=(60, 24, 22), steps=25, form='fp32'):
     mask = _car_mask(dims, seed=8)
-    t = _tunnel(dims, 'fp32', mask, 0.05)
+    t = _tunnel(dims, form, mask, 0.05)
     t.step(steps, 0.05, 0.52, 0.17)
     return t, mask
 
 
-def test_advect_bit_exact(gpu):
-    t, mask = _flow_state()
+# FP16 tunnels keep their macroscopic field as binary16 (ux, uy, uz, rho - 1);
+# the particle, slice and Cp kernels widen it as the oracle's device_field does
+@pytest.mark.parametrize('form', ['fp32', 'fp16'])
+def test_advect_bit_exact(gpu, form):
+    t, mask = _flow_state(form=form)
     dims = (t.nx, t.ny, t.nz)
     rng = np.random.default_rng(3)
     N = 20000
@@ -389,9 +392,10 @@ def test_pack_bit_exact(gpu):
     assert np.array_equal(q8.cpu().numpy(), e8)
 
 
+@pytest.mark.parametrize('form', ['fp32', 'fp16'])
 @pytest.mark.parametrize('axis,field', [(2, 0), (2, 1), (2, 2), (2, 3), (1, 2), (0, 0), (0, 2)])
-def test_slice_bit_exact(gpu, axis, field):
-    t, mask = _flow_state()
+def test_slice_bit_exact(gpu, axis, field, form):
+    t, mask = _flow_state(form=form)
     dims = (t.nx, t.ny, t.nz)
     ranges = {0: (0, 1.6), 1: (-1.2, 1.2), 2: (0, 12), 3: (-0.4, 1.4)}
     vmin, vmax = ranges[field]
@@ -403,8 +407,9 @@ def test_slice_bit_exact(gpu, axis, field):
     assert np.array_equal(out.cpu().numpy(), want)
 
 
-def test_cp_bit_exact(gpu):
-    t, mask = _flow_state()
+@pytest.mark.parametrize('form', ['fp32', 'fp16'])
+def test_cp_bit_exact(gpu, form):
+    t, mask = _flow_state(form=form)
     dims = (t.nx, t.ny, t.nz)
     rng = np.random.default_rng(2)
     nv = 3000

@@ -106,13 +106,14 @@ def test_buffer_sizes_and_footprint(form, elem):
     n = nx * ny * nz
     s = (ctypes.c_int64 * 6)()
     assert lib.wt_buffer_sizes(nx, ny, nz, form, s) == 0
-    assert s[0] == 19 * n * elem and s[1] == n and s[2] == 16 * n
+    mac = 8 if form in (1, 2) else 16          # binary16 x 4 for FP16/BF16, float4 otherwise
+    assert s[0] == 19 * n * elem and s[1] == n and s[2] == mac * n
     shell = 2 * (ny * nz + nx * nz + nx * ny) * 4 * (8 if form == 3 else 4)
     assert s[3] == shell
     total = lib.wt_state_bytes(nx, ny, nz, form)
     assert total == 2 * s[0] + s[1] + s[2] + 2 * s[3] + s[4] + s[5]
-    # per cell: 2 x 19 populations + 16 B mac + 1 B class, plus O(n^(2/3)) extras
-    per_cell = 2 * 19 * elem + 17
+    # per cell: 2 x 19 populations + the mac entry + 1 B class, plus O(n^(2/3)) extras
+    per_cell = 2 * 19 * elem + mac + 1
     assert per_cell * n <= total <= per_cell * n * 1.02
     if form == 0:
         assert per_cell == 169      # lbm.py:30 BYTES_PER_CELL

@@ -29,7 +29,7 @@ def test_tick_budget_admission_rule():
     from windtunnel.serving import FORM_COST, TickBudget
     tb = TickBudget(10.0, sec_per_unit=1e-9, adapt=False)        # 10 ms = 1e7 units
     assert tb.units((100, 100, 100), 'fp32') == 1e6
-    assert tb.units((100, 100, 100), 'fp16') == 1e6 * 93 / 169
+    assert tb.units((100, 100, 100), 'fp16') == 1e6 * 85 / 169
     assert FORM_COST['fp64'] == 321 / 169
     assert tb.admits(0.0, 5e8)                                    # an idle GPU takes any head
     assert tb.admits(9e6, 1e6) and not tb.admits(9e6, 1.5e6)

@@ -76,7 +76,7 @@ __global__ __launch_bounds__(ABLOCK) void k_pack(const float4* __restrict__ pos,
 }
 
 struct SliceArgs {
-  const float4* mac;
+  MacView mac;
   const uint8_t* cls;
   uint8_t* out;
   int nx, ny, nz, axis, index, field, w;
@@ -118,7 +118,7 @@ __global__ __launch_bounds__(ABLOCK) void k_slice(SliceArgs a) {
 }
 
 // Fluid-only trilinear density, lbm.py:321-350.
-__device__ float2 rho_fluid(const float4* __restrict__ mac, const uint8_t* __restrict__ cls, float px,
+__device__ float2 rho_fluid(const MacView& mac, const uint8_t* __restrict__ cls, float px,
                             float py, float pz, int nx, int ny, int nz) {
   const float qx = clampf(px, 0.0f, (float)nx - 1.001f);
   const float qy = clampf(py, 0.0f, (float)ny - 1.001f);
@@ -143,7 +143,7 @@ __device__ float2 rho_fluid(const float4* __restrict__ mac, const uint8_t* __res
 }
 
 struct CpArgs {
-  const float4* mac;
+  MacView mac;
   const uint8_t* cls;
   const float* verts;
   const float* norms;
@@ -256,7 +256,7 @@ int wt_advect(const wt_grid* g, float* pos, float* spd, int64_t num_p, float dt,
   if (num_p < 0 || num_p >= ((int64_t)1 << 31)) return fail(-1, "bad particle count %lld", (long long)num_p);
   if (num_p == 0) return 0;
   if (!pos || !spd || !box) return fail(-1, "null particle buffer or seed box");
-  AdvectArgs a{reinterpret_cast<const float4*>(g->mac), g->cls, nullptr, nullptr, reinterpret_cast<float4*>(pos),
+  AdvectArgs a{MacView{g->mac, mac_is_half(g->form)}, g->cls, nullptr, nullptr, reinterpret_cast<float4*>(pos),
                spd, (uint32_t)num_p, g->nx, g->ny, g->nz, dt, uin, max_age, seed, *box};
   hipLaunchKernelGGL(k_advect, dim3((unsigned)((num_p + ABLOCK - 1) / ABLOCK)), dim3(ABLOCK), 0,
                      (hipStream_t)stream, a);
@@ -269,7 +269,7 @@ int wt_advect_ordered(const wt_grid* g, float* pos, float* spd, const uint32_t* 
   if (num_p < 0 || num_p >= ((int64_t)1 << 31)) return fail(-1, "bad particle count %lld", (long long)num_p);
   if (num_p == 0) return 0;
   if (!pos || !spd || !box || !order) return fail(-1, "null particle buffer, order or seed box");
-  AdvectArgs a{reinterpret_cast<const float4*>(g->mac), g->cls, order, nullptr, reinterpret_cast<float4*>(pos),
+  AdvectArgs a{MacView{g->mac, mac_is_half(g->form)}, g->cls, order, nullptr, reinterpret_cast<float4*>(pos),
                spd, (uint32_t)num_p, g->nx, g->ny, g->nz, dt, uin, max_age, seed, *box};
   hipLaunchKernelGGL(k_advect, dim3((unsigned)((num_p + ABLOCK - 1) / ABLOCK)), dim3(ABLOCK), 0,
                      (hipStream_t)stream, a);
@@ -282,7 +282,7 @@ int wt_advect_slots(const wt_grid* g, float* pos, float* spd, const uint32_t* id
   if (num_p < 0 || num_p >= ((int64_t)1 << 31)) return fail(-1, "bad particle count %lld", (long long)num_p);
   if (num_p == 0) return 0;
   if (!pos || !spd || !box || !ids) return fail(-1, "null particle buffer, ids or seed box");
-  AdvectArgs a{reinterpret_cast<const float4*>(g->mac), g->cls, nullptr, ids, reinterpret_cast<float4*>(pos),
+  AdvectArgs a{MacView{g->mac, mac_is_half(g->form)}, g->cls, nullptr, ids, reinterpret_cast<float4*>(pos),
                spd, (uint32_t)num_p, g->nx, g->ny, g->nz, dt, uin, max_age, seed, *box};
   hipLaunchKernelGGL(k_advect, dim3((unsigned)((num_p + ABLOCK - 1) / ABLOCK)), dim3(ABLOCK), 0,
                      (hipStream_t)stream, a);
@@ -381,7 +381,7 @@ int wt_slice(const wt_grid* g, uint8_t* out, int axis, int index, int field, flo
   // slice_dims, lbm.py:455-456
   const int w = axis == 2 ? g->nx : (axis == 1 ? g->nx : g->nz);
   const int h = axis == 2 ? g->ny : (axis == 1 ? g->nz : g->ny);
-  SliceArgs a{reinterpret_cast<const float4*>(g->mac), g->cls, out, g->nx, g->ny, g->nz, axis, index, field,
+  SliceArgs a{MacView{g->mac, mac_is_half(g->form)}, g->cls, out, g->nx, g->ny, g->nz, axis, index, field,
               w, (uint32_t)w * (uint32_t)h, uin, lref, vmin, vmax};
   hipLaunchKernelGGL(k_slice, dim3((a.npix + ABLOCK - 1) / ABLOCK), dim3(ABLOCK), 0, (hipStream_t)stream, a);
   return check_launch("wt_slice");
@@ -393,7 +393,7 @@ int wt_cp(const wt_grid* g, const float* verts, const float* norms, int8_t* out,
   if (nv < 0 || nv >= ((int64_t)1 << 31)) return fail(-1, "bad vertex count %lld", (long long)nv);
   if (nv == 0) return 0;
   if (!verts || !norms || !out) return fail(-1, "null vertex, normal or output buffer");
-  CpArgs a{reinterpret_cast<const float4*>(g->mac), g->cls, verts, norms, out, (uint32_t)nv,
+  CpArgs a{MacView{g->mac, mac_is_half(g->form)}, g->cls, verts, norms, out, (uint32_t)nv,
            g->nx, g->ny, g->nz, uin, cp_range};
   hipLaunchKernelGGL(k_cp, dim3((unsigned)((nv + ABLOCK - 1) / ABLOCK)), dim3(ABLOCK), 0, (hipStream_t)stream, a);
   return check_launch("wt_cp");

@@ -122,6 +122,29 @@ template <> struct Form<WT_FP64> {
   __device__ static double excess(elem e, int i) { return e - wd(i); }
 };
 
+// ---- the macroscopic field ------------------------------------------------------
+// FP32 and FP64 tunnels store (ux, uy, uz, rho) as float4 per cell (lbm.py:406);
+// FP16 and BF16 tunnels store it as 4 binary16 values (ux, uy, uz, rho - 1),
+// each rounded to nearest even from FP32, 8 B per cell instead of 16. Readers
+// rebuild float4 (ux, uy, uz, 1 + (rho - 1)) in FP32. The step never reads it
+// (face cells read the FP32 face-layer snapshot), so its precision changes no
+// population; particles, slices and Cp read it.
+inline bool mac_is_half(int form) { return form == WT_FP16 || form == WT_BF16; }
+inline int mac_elem_bytes(int form) { return mac_is_half(form) ? 8 : 16; }
+
+struct MacView {
+  const void* p;
+  int half;
+  __device__ __forceinline__ float4 operator[](uint32_t idx) const {
+    if (half) {
+      const uint2 v = reinterpret_cast<const uint2*>(p)[idx];
+      return make_float4(h2f((uint16_t)(v.x & 0xffffu)), h2f((uint16_t)(v.x >> 16)), h2f((uint16_t)(v.y & 0xffffu)),
+                         1.0f + h2f((uint16_t)(v.y >> 16)));
+    }
+    return reinterpret_cast<const float4*>(p)[idx];
+  }
+};
+
 inline int form_elem_bytes(int form) { return form == WT_FP32 ? 4 : (form == WT_FP64 ? 8 : 2); }
 inline int form_real_bytes(int form) { return form == WT_FP64 ? 8 : 4; }
 
@@ -148,7 +171,7 @@ struct StepDesc {
   const void* fin;
   void* fout;
   const uint8_t* cls;
-  float4* mac;
+  void* mac;          // float4 or 4 x binary16 per cell by storage form (MacView)
   const void* shell_in;
   void* shell_out;
   double* partials;   // per-tile slots of 4 doubles, or null

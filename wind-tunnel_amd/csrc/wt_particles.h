@@ -35,7 +35,7 @@ struct Corners {
   float4 c[8];   // (x, y, z) offsets 000 100 010 110 001 101 011 111
 };
 
-__device__ __forceinline__ float4 sample_mac(const float4* __restrict__ mac, float px, float py, float pz, int nx,
+__device__ __forceinline__ float4 sample_mac(const MacView& mac, float px, float py, float pz, int nx,
                                              int ny, int nz, Corners& k, bool have) {
   const float qx = clampf(px, 0.0f, (float)nx - 1.001f);
   const float qy = clampf(py, 0.0f, (float)ny - 1.001f);
@@ -71,7 +71,7 @@ __device__ __forceinline__ float rng_f(uint32_t& st) {
 typedef float f32x4p __attribute__((ext_vector_type(4)));
 
 struct AdvectArgs {
-  const float4* mac;
+  MacView mac;
   const uint8_t* cls;
   const uint32_t* order;  // processing order (a permutation of particle slots) or null
   const uint32_t* ids;    // particle id held in each slot (slot storage) or null (slot = id)

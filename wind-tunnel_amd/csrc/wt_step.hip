@@ -88,10 +88,22 @@ template <bool NT, typename E> __device__ __forceinline__ void st_stream(E* p, E
 }
 
 typedef float f32x4 __attribute__((ext_vector_type(4)));
-template <bool NT>
-__device__ __forceinline__ void st_mac(float4* p, float a, float b, float c, float d) {
-  if constexpr (NT) __builtin_nontemporal_store(f32x4{a, b, c, d}, reinterpret_cast<f32x4*>(p));
-  else *p = make_float4(a, b, c, d);
+typedef uint32_t u32x2 __attribute__((ext_vector_type(2)));
+// The macroscopic field of cell idx in its storage form's layout (MacView):
+// float4 (ux, uy, uz, rho), or for FP16/BF16 binary16 (ux, uy, uz, rho - 1).
+template <int FORM, bool NT>
+__device__ __forceinline__ void st_mac(void* mac, uint32_t idx, float a, float b, float c, float d) {
+  if constexpr (Form<FORM>::dev) {
+    const uint32_t lo = (uint32_t)f2h(a) | ((uint32_t)f2h(b) << 16);
+    const uint32_t hi = (uint32_t)f2h(c) | ((uint32_t)f2h(d - 1.0f) << 16);
+    u32x2* p = reinterpret_cast<u32x2*>(mac) + idx;
+    if constexpr (NT) __builtin_nontemporal_store(u32x2{lo, hi}, p);
+    else *p = u32x2{lo, hi};
+  } else {
+    f32x4* p = reinterpret_cast<f32x4*>(mac) + idx;
+    if constexpr (NT) __builtin_nontemporal_store(f32x4{a, b, c, d}, p);
+    else *p = f32x4{a, b, c, d};
+  }
 }
 
 // Signed sum c_i . u in index order, as cu_expr() emits it (lbm.py:48-53).
@@ -581,7 +593,7 @@ __device__ __forceinline__ void step_cell_general(const StepDesc& d, const typen
   if (__any((c & CLS_NEAR) != 0)) stage_link_rows(rows, d.cls, d.n, idx0, nx, nxy);
 
   if ((c & CLS_CODE) != 0) {  // solid cell, lbm.py:81-83
-    st_mac<POL == 1>(d.mac + idx, 0.f, 0.f, 0.f, 1.f);
+    st_mac<FORM, POL == 1>(d.mac, idx, 0.f, 0.f, 0.f, 1.f);
     shell_write<R>(shout, x, y, z, zero, zero, zero, one);
     return;
   }
@@ -615,7 +627,7 @@ __device__ __forceinline__ void step_cell_general(const StepDesc& d, const typen
 #pragma unroll
     for (int i = 0; i < Q; i++) v[i] = S::store(face_feq<R>(i, br, bux, buy, buz, bsq), i);
     store_all(v);
-    st_mac<POL == 1>(d.mac + idx, (float)bux, (float)buy, (float)buz, (float)br);
+    st_mac<FORM, POL == 1>(d.mac, idx, (float)bux, (float)buy, (float)buz, (float)br);
     if (flags & WT_FLAG_MASS) acc.m += (double)br;
     return;
   }
@@ -670,7 +682,7 @@ __device__ __forceinline__ void step_cell_general(const StepDesc& d, const typen
 #pragma unroll
   for (int i = 0; i < Q; i++) st[i] = S::put(out[i]);
   store_all(st);
-  st_mac<POL == 1>(d.mac + idx, (float)ux, (float)uy, (float)uz, (float)rho);
+  st_mac<FORM, POL == 1>(d.mac, idx, (float)ux, (float)uy, (float)uz, (float)rho);
   shell_write<R>(shout, x, y, z, ux, uy, uz, rho);
   if (flags & WT_FLAG_MASS) acc.m += (double)rho;
   if ((flags & WT_FLAG_FORCES) && (c & CLS_FORCE)) add_force<FORM>(staged_force_links(rows, idx0, nx, nxy), st, acc);
@@ -747,7 +759,7 @@ __device__ __forceinline__ void step_cell_fast(const StepDesc& d, const typename
   } else {
     global_stores();
   }
-  st_mac<POL == 1>(d.mac + idx, (float)ux, (float)uy, (float)uz, (float)rho);
+  st_mac<FORM, POL == 1>(d.mac, idx, (float)ux, (float)uy, (float)uz, (float)rho);
   if (!xface) shell_write<R>(shout, x, y, z, ux, uy, uz, rho);
   if (flags & WT_FLAG_MASS) acc.m += (double)rho;
 }
@@ -1032,7 +1044,7 @@ __global__ __launch_bounds__(BLOCK) void k_force(StepDesc d) {
 // ---- init (lbm.py:64-69) ----------------------------------------------------------------
 template <int FORM>
 __global__ __launch_bounds__(BLOCK) void k_init(typename Form<FORM>::elem* f0,
-                                                typename Form<FORM>::elem* f1, float4* mac,
+                                                typename Form<FORM>::elem* f1, void* mac,
                                                 uint32_t n, uint32_t nx, float uin_f) {
   typedef Form<FORM> S;
   typedef typename S::real R;
@@ -1048,7 +1060,7 @@ __global__ __launch_bounds__(BLOCK) void k_init(typename Form<FORM>::elem* f0,
     f0[pc + (uint32_t)ppos(i) * nx] = e;
     f1[pc + (uint32_t)ppos(i) * nx] = e;
   }
-  mac[idx] = make_float4(uin_f, 0.f, 0.f, 1.f);
+  st_mac<FORM, false>(mac, idx, uin_f, 0.f, 0.f, 1.f);
 }
 
 template <typename R>
@@ -1214,7 +1226,7 @@ StepDesc make_desc(const wt_grid* g, int cur, const wt_params* p, int flags) {
   d.fin = g->f[cur & 1];
   d.fout = g->f[(cur & 1) ^ 1];
   d.cls = g->cls;
-  d.mac = reinterpret_cast<float4*>(g->mac);
+  d.mac = g->mac;
   d.shell_in = g->shell[cur & 1];
   d.shell_out = g->shell[(cur & 1) ^ 1];
   d.partials = g->partials;
@@ -1343,7 +1355,7 @@ int wt_buffer_sizes(int nx, int ny, int nz, int form, int64_t sizes[6]) {
   const int64_t n = (int64_t)nx * ny * nz;
   sizes[0] = (int64_t)Q * n * form_elem_bytes(form);
   sizes[1] = n;
-  sizes[2] = 16 * n;
+  sizes[2] = mac_elem_bytes(form) * n;
   sizes[3] = 4 * shell_entries(nx, ny, nz) * form_real_bytes(form);
   sizes[4] = 32 * (nslots_of(n) + nchunks_of(n));
   sizes[5] = 32;
@@ -1363,7 +1375,7 @@ int wt_init(const wt_grid* g, float uin, void* stream) {
   hipStream_t s = (hipStream_t)stream;
   const uint32_t n = (uint32_t)((int64_t)g->nx * g->ny * g->nz);
   const dim3 grid((n + BLOCK - 1) / BLOCK);
-  float4* mac = reinterpret_cast<float4*>(g->mac);
+  void* mac = g->mac;
   const uint32_t ent = (uint32_t)shell_entries(g->nx, g->ny, g->nz);
   const dim3 sgrid((ent + BLOCK - 1) / BLOCK);
   switch (g->form) {
@@ -1437,7 +1449,7 @@ int wt_step_noalloc(const wt_grid* g, int cur, int nsteps, const wt_params* p, i
   return wt_step(&h, cur, nsteps, p, flags, stream);
 }
 
-int wt_step_advect(const wt_grid* g, int cur, const wt_params* p, int flags, const float* mac_prev, float* pos,
+int wt_step_advect(const wt_grid* g, int cur, const wt_params* p, int flags, const void* mac_prev, float* pos,
                    float* spd, const uint32_t* ids, int64_t num_p, float dt, float uin, float max_age,
                    uint32_t seed, const wt_seed_box* box, void* stream) {
   if (int e = check_grid(g, true)) return e;
@@ -1446,7 +1458,7 @@ int wt_step_advect(const wt_grid* g, int cur, const wt_params* p, int flags, con
     return fail(-1, "reduction flags need partials and sums buffers");
   if (num_p < 0 || num_p >= ((int64_t)1 << 31)) return fail(-1, "bad particle count %lld", (long long)num_p);
   if (num_p > 0 && (!pos || !spd || !box || !mac_prev)) return fail(-1, "null particle buffer, field or seed box");
-  if (num_p > 0 && (const void*)mac_prev == (const void*)g->mac)
+  if (num_p > 0 && mac_prev == (const void*)g->mac)
     return fail(-1, "the advected field must not be the grid's mac buffer (the step writes it)");
   Shape sh;
   if (int e = grid_shape(g, &sh)) return e;
@@ -1456,7 +1468,7 @@ int wt_step_advect(const wt_grid* g, int cur, const wt_params* p, int flags, con
   sh.remap = (sh.remap & 1) | (((sh.remap & 2) && (cur & 1)) ? 2 : 0);   // as step k = 0 of wt_step
   sh.solo = solo_policy(g);
   const wt_seed_box none{0.f, 0.f, 0.f, 0.f, 0.f};
-  const AdvectArgs a{reinterpret_cast<const float4*>(mac_prev), g->cls, nullptr, ids, reinterpret_cast<float4*>(pos),
+  const AdvectArgs a{MacView{mac_prev, mac_is_half(g->form)}, g->cls, nullptr, ids, reinterpret_cast<float4*>(pos),
                      spd, (uint32_t)num_p, g->nx, g->ny, g->nz, dt, uin, max_age, seed, box ? *box : none};
   const int f = flags & (WT_FLAG_FORCES | WT_FLAG_MASS);
   switch (g->form) {

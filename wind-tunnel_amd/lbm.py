@@ -91,7 +91,7 @@ class Solver:
         self.T = Tunnel(dims, storage)
         self.storage = storage
         self.solid = self.T.cls          # device class bytes (low 2 bits: solid value)
-        self.mac = self.T.mac            # device float4 (ux, uy, uz, rho)
+        self.mac = self.T.mac            # device (ux, uy, uz, rho): float4, or float16 with rho - 1 (fp16/bf16)
         self.slice_max = max(self.nx * self.ny, self.nx * self.nz, self.nz * self.ny)
         self.slice_buf = torch.zeros(self.slice_max * 2, dtype=torch.uint8, device='cuda')
         self.num_p = 0

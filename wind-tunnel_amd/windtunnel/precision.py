@@ -56,7 +56,7 @@ def field_errors(scale=3, step=200, forms=FORMS):
     for form in ('fp64',) + tuple(f for f in forms if f != 'fp64'):
         t = _tunnel(dims, mask, form)
         t.step(step, UIN, TAU, SMAG)
-        m = t.mac[fluid].double()
+        m = t.macroscopic()[fluid].double()
         fields[form] = (m[:, :3].clone(), (m[:, 3] - 1.0) / (1.5 * UIN * UIN))
         del t, m
         torch.cuda.empty_cache()

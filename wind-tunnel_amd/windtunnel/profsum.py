@@ -21,7 +21,7 @@ import os
 import re
 import sys
 
-BYTES_PER_CELL = {0: 169, 1: 93, 2: 93, 3: 321}
+BYTES_PER_CELL = {0: 169, 1: 85, 2: 85, 3: 321}
 BYTES_PER_PARTICLE = 40   # id 4, position 16 read + 16 written, speed 4 written
 FORM_NAMES = {0: 'fp32', 1: 'fp16', 2: 'bf16', 3: 'fp64'}
 

@@ -9,7 +9,8 @@ Each form runs the launch shape tuned for its size class (windtunnel/tuning.json
 as sessions and bench.py do), or with --sweep a list of shapes. Prints one JSON
 object per measurement. Algorithmic bytes per cell of the step:
 19 populations read + 19 written, 1 class byte, 16 B macroscopic store
-(FP32 169, FP16/BF16 93, FP64 321). With --particles, the forms named by
+(FP32 169, FP64 321: 16 B float4 macroscopic field; FP16/BF16 85: 2-byte
+deviations and an 8 B binary16 macroscopic field). With --particles, the forms named by
 --fused are timed as wt_step_advect launches (the step fused with the slot
 advection of N particles seeded on the inlet plane), priced at 40 more bytes
 per particle: id 4, position 16 read + 16 written, speed 4 written.
@@ -25,7 +26,7 @@ import torch
 from . import _lib, autotune, geometry
 from .tunnel import Tunnel
 
-BYTES_PER_CELL = {'fp32': 169, 'fp16': 93, 'bf16': 93, 'fp64': 321}
+BYTES_PER_CELL = {'fp32': 169, 'fp16': 85, 'bf16': 85, 'fp64': 321}
 BYTES_PER_PARTICLE = 40
 PEAK_GBS = 8000.0
 

@@ -52,8 +52,8 @@ P99_TARGET_MS = 1000.0 / 30.0
 TICK_TARGET_MS = 30.0
 
 # Relative cost of one cell update per storage form: the step is bound by HBM
-# traffic, so cost follows bytes per cell (169 FP32, 93 FP16/BF16, 321 FP64).
-FORM_COST = {'fp32': 1.0, 'fp16': 93 / 169, 'bf16': 93 / 169, 'fp64': 321 / 169}
+# traffic, so cost follows bytes per cell (169 FP32, 85 FP16/BF16, 321 FP64).
+FORM_COST = {'fp32': 1.0, 'fp16': 85 / 169, 'bf16': 85 / 169, 'fp64': 321 / 169}
 
 
 class TickBudget:

@@ -3,8 +3,8 @@
 torch provides the device buffers and the stream; every computation is a
 libwt call. Buffer layout (see include/wt.h): two row-blocked SoA
 population buffers f[0], f[1] (each x-row holds its 19 planes) of 19*n elements, a class byte per cell, the
-float4 macroscopic field mac (ux, uy, uz, rho), two face-layer snapshots, and
-the FP64 reduction slots.
+macroscopic field mac (float32 ux, uy, uz, rho; binary16 ux, uy, uz, rho - 1 for
+FP16/BF16 tunnels), two face-layer snapshots, and the FP64 reduction slots.
 """
 import ctypes
 
@@ -15,6 +15,9 @@ from . import _lib, autotune
 from ._lib import Grid, Params, SeedBox, check, stream_handle, FORMS, FORM_NAMES
 
 ELEM_DTYPE = {0: torch.float32, 1: torch.int16, 2: torch.int16, 3: torch.float64}
+# the macroscopic field per cell: float32 (ux, uy, uz, rho) for FP32/FP64 tunnels,
+# float16 (ux, uy, uz, rho - 1) for FP16/BF16 tunnels (include/wt.h)
+MAC_DTYPE = {0: torch.float32, 1: torch.float16, 2: torch.float16, 3: torch.float32}
 # position of direction i's plane inside a row block (include/wt.h)
 PLANE_POS = [0, 1, 2, 3, 6, 9, 12, 4, 7, 8, 5, 10, 13, 14, 11, 15, 16, 17, 18]
 # population weights w_i (server/lbm.py:18)
@@ -64,7 +67,7 @@ class Tunnel:
         dev = self.device
         self.f = [torch.empty(sz[0], dtype=torch.uint8, device=dev) for _ in range(2)]
         self.cls = torch.zeros(sz[1], dtype=torch.uint8, device=dev)
-        self.mac = torch.zeros((self.n, 4), dtype=torch.float32, device=dev)
+        self.mac = torch.zeros((self.n, 4), dtype=MAC_DTYPE[self.form], device=dev)
         self.shell = [torch.empty(sz[3], dtype=torch.uint8, device=dev) for _ in range(2)]
         self.partials = torch.zeros(sz[4] // 8, dtype=torch.float64, device=dev)
         self.sums = torch.zeros(4, dtype=torch.float64, device=dev)
@@ -156,10 +159,11 @@ class Tunnel:
             check(self.lib.wt_set_solid(self.gp, _ptr(dev), stream_handle(s)), 'wt_set_solid')
 
     def set_mac(self, mac):
-        """Point later launches at another (n, 4) float32 macroscopic buffer. The
-        step kernel only writes mac (face cells read the shell snapshot), so a
-        caller may alternate two buffers and read one while a step writes the other."""
-        self._arg(mac, 'mac', F32, (self.n, 4))
+        """Point later launches at another (n, 4) macroscopic buffer (float32, or
+        float16 for FP16/BF16 tunnels). The step kernel only writes mac (face cells
+        read the shell snapshot), so a caller may alternate two buffers and read one
+        while a step writes the other."""
+        self._arg(mac, 'mac', (MAC_DTYPE[self.form],), (self.n, 4))
         self.mac = mac
         self.grid.mac = mac.data_ptr()
 
@@ -175,7 +179,7 @@ class Tunnel:
         previous step from mac_prev, the field that step wrote (not self.mac)."""
         p = Params(float(uin), float(tau), float(smag))
         sb = SeedBox(*(float(v) for v in box))
-        mp = self._arg(mac_prev, 'mac_prev', F32, (self.n, 4))
+        mp = self._arg(mac_prev, 'mac_prev', (MAC_DTYPE[self.form],), (self.n, 4))
         n, (pp, sp, ip) = self._particles(pos, spd, ids=ids)
         check(self.lib.wt_step_advect(self.gp, self.cur, ctypes.byref(p), int(flags), mp, pp, sp, ip, n, float(dt),
                                       float(uin), float(max_age), int(seed) & 0xffffffff, ctypes.byref(sb),
@@ -193,6 +197,16 @@ class Tunnel:
         return out
 
     # ---- views ---------------------------------------------------------------
+    def macroscopic(self, mac=None):
+        """(n, 4) float32 (ux, uy, uz, rho) on the device as the particle, slice and
+        Cp kernels read it: FP16/BF16 tunnels store (ux, uy, uz, rho - 1) in float16,
+        and rho is rebuilt as 1 + (rho - 1) in float32."""
+        m = self.mac if mac is None else mac
+        if m.dtype == torch.float16:
+            f = m.float()
+            f[:, 3] += 1.0
+            return f
+        return m
     def raw_populations(self, which=None):
         """f[which] (default f[cur]) as its storage dtype, shape (19, n) in the
         reference's plane order (population i of cell idx at [i, idx]), on the

@@ -23,8 +23,8 @@ macroscopic store) over its mean launch time, from HIP events on
 the stream the kernel runs on, bracketing every 5th launch of the timed
 region (session steps 0 mod 5: force-free; a pair of events on every step slows
 the loop by ~0.7 %). Each step's particles are advected inside the next
-step's launch (k_step_adv, the sessions' default: 0.15 % faster per step than
-a separate advection launch for FP32, 1.4 % for FP16), so that kernel is priced
+step's launch (k_step_adv, the sessions' default: 0.6 % faster per step than
+a separate advection launch for FP16, the same for FP32), so that kernel is priced
 at 40 more bytes per particle (id 4, position 16 read + 16 written, speed 4
 written); the last step's particles are advected by their own launch at the
 end of the timed region.

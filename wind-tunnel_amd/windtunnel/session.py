@@ -20,11 +20,10 @@ from .coefficients import ForceAverage
 from .tunnel import Tunnel
 
 # storage forms whose sessions fuse the particle advection into the next step's
-# launch by default (Session(fuse=...) overrides). With the clamp's square root
-# taken only where it can fire and the deviation-form FP16/BF16 collision, the
-# fused loop measured 0.15 % (FP32) and 1.4 % (FP16) faster per step than a
-# separate advection launch at 33.5M cells and 2^20 particles, three
-# alternating same-process repeats (0.9765 vs 0.9780 ms, 0.5796 vs 0.5880 ms).
+# launch by default (Session(fuse=...) overrides). Against a separate advection
+# launch at 33.5M cells and 2^20 particles (three alternating same-process
+# repeats of 200-step loops): FP16 0.6 % faster per step (0.5525 vs 0.5560 ms),
+# FP32 the same within the repeats' spread (0.9797 vs 0.9780 ms).
 FUSED_FORMS = ('fp32', 'fp16', 'bf16', 'fp64')
 # frames whose forces the device-side log holds between two reads of the average
 FRAME_LOG = 4096

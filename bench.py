@@ -22,12 +22,13 @@ float4 macroscopic store; 85 B/cell FP16: 2-byte deviations and an 8 B binary16
 macroscopic store) over its mean launch time, from HIP events on
 the stream the kernel runs on, bracketing every 5th launch of the timed
 region (session steps 0 mod 5: force-free; a pair of events on every step slows
-the loop by ~0.7 %). Each step's particles are advected inside the next
-step's launch (k_step_adv, the sessions' default: 0.6 % faster per step than
-a separate advection launch for FP16, the same for FP32), so that kernel is priced
-at 40 more bytes per particle (id 4, position 16 read + 16 written, speed 4
-written); the last step's particles are advected by their own launch at the
-end of the timed region.
+the loop by ~0.7 %). FP32 sessions advect each step's particles with their
+own launch after the step (k_advect); FP16 sessions advect them inside the next
+step's launch (k_step_adv: 0.4-0.6 % faster per step for FP16, for FP32 the
+separate launches are 0.05-0.2 % faster), so that kernel is priced at 40 more
+bytes per particle (id 4, position 16 read + 16 written, speed 4 written); the
+last step's particles are advected by their own launch at the end of the
+timed region.
 cpu_baseline: the CPU restatement of the same step (oracle/, C++ with OpenMP)
 timed on this host on a bounded sample.
 """

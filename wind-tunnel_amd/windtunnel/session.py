@@ -21,10 +21,11 @@ from .tunnel import Tunnel
 
 # storage forms whose sessions fuse the particle advection into the next step's
 # launch by default (Session(fuse=...) overrides). Against a separate advection
-# launch at 33.5M cells and 2^20 particles (three alternating same-process
-# repeats of 200-step loops): FP16 0.6 % faster per step (0.5525 vs 0.5560 ms),
-# FP32 the same within the repeats' spread (0.9797 vs 0.9780 ms).
-FUSED_FORMS = ('fp32', 'fp16', 'bf16', 'fp64')
+# launch at 33.5M cells and 2^20 particles (same process, 200-step loops, two
+# runs of 3 and 6 alternating repeats): FP16 0.4-0.6 % faster per step fused
+# (0.5491 vs 0.5516 ms); FP32 0.05-0.2 % faster separate (0.9742 vs 0.9747 ms),
+# so FP32 and FP64 sessions keep the step kernel and the advection apart.
+FUSED_FORMS = ('fp16', 'bf16')
 # frames whose forces the device-side log holds between two reads of the average
 FRAME_LOG = 4096
 

@@ -31,6 +31,15 @@ extern "C" {
 /* wt_step / wt_batch_step flags */
 #define WT_FLAG_FORCES 1 /* fuse the momentum-exchange force into the last step */
 #define WT_FLAG_MASS 2   /* fuse the FP64 mass reduction into the last step */
+/* Leave the macroscopic field unwritten in every step of the call (mac keeps
+ * the field of the last step that stored it). The field is an output only:
+ * face cells read the shell snapshot, forces come from the populations, so
+ * populations, shells and sums are the same bits with or without the flag. For
+ * steps whose field nothing reads: no particle advection, slice or surface
+ * pressure from it (the reference's Solver.step reads it once, after the last
+ * of its steps: lbm.py:443-453). In a batch, tunnels whose gate byte is set
+ * (wt_batch_gate_offset) still store it. No counterpart in the reference. */
+#define WT_FLAG_NO_MAC 4
 
 /* One tunnel's device state. n = nx*ny*nz; sizes come from wt_buffer_sizes().
  * f[0], f[1]: ping-pong populations, row-blocked SoA: each x-row
@@ -122,10 +131,11 @@ int wt_set_solid(const wt_grid* g, const uint8_t* solid_dev, void* stream);
 
 /* ---- the hot path --------------------------------------------------------- */
 /* nsteps fused D3Q19 steps starting from f[cur] (reads f[cur], writes
- * f[cur^1], and so on). The caller flips its own cur nsteps times. flags apply
- * to the last step; with WT_FLAG_FORCES or WT_FLAG_MASS the reduced values are
- * in g->sums when the stream reaches the end of this call. Replaces the step
- * loop of Solver.step (lbm.py:442-447) over step_kernel (lbm.py:72-166). */
+ * f[cur^1], and so on). The caller flips its own cur nsteps times.
+ * WT_FLAG_FORCES and WT_FLAG_MASS apply to the last step, and the reduced
+ * values are in g->sums when the stream reaches the end of this call;
+ * WT_FLAG_NO_MAC applies to every step. Replaces the step loop of Solver.step
+ * (lbm.py:442-447) over step_kernel (lbm.py:72-166). */
 int wt_step(const wt_grid* g, int cur, int nsteps, const wt_params* p, int flags, void* stream);
 /* wt_step without allocation of any kind, for callers that cannot allocate
  * device memory (code under HIP graph capture, tight serving loops, harnesses
@@ -142,7 +152,9 @@ int wt_step_noalloc(const wt_grid* g, int cur, int nsteps, const wt_params* p, i
  * macroscopic field that step wrote. mac_prev must not be g->mac, which this
  * step overwrites: callers alternate two mac buffers. Removes one kernel
  * boundary per step (lbm.py:442-453 launches the step and advect_kernel
- * separately); every result equals the two separate launches bit for bit. */
+ * separately); every result equals the two separate launches bit for bit.
+ * flags: WT_FLAG_FORCES and WT_FLAG_MASS (the next step's advection reads the
+ * field this step stores, so WT_FLAG_NO_MAC is rejected). */
 int wt_step_advect(const wt_grid* g, int cur, const wt_params* p, int flags, const void* mac_prev, float* pos,
                    float* spd, const uint32_t* ids, int64_t num_p, float dt, float uin, float max_age,
                    uint32_t seed, const wt_seed_box* box, void* stream);
@@ -169,8 +181,16 @@ typedef struct wt_batch_info {
   int32_t tile_order;
   int32_t reserved[2];
 } wt_batch_info;
-/* Device workspace bytes for a batch of ntunnels (descriptor tables). */
+/* Device workspace bytes for a batch of ntunnels (descriptor tables and the
+ * per-tunnel gate bytes). */
 int64_t wt_batch_workspace_bytes(int ntunnels);
+/* Byte offset in the workspace of the batch's ntunnels gate bytes: with
+ * WT_FLAG_NO_MAC, wt_batch_step stores the macroscopic field of tunnel k only
+ * if gate byte k is non-zero (the encoded table sets every gate to 1). A
+ * caller writes them in stream order before the step whose field it will read
+ * (the serving loop: the sessions whose slice is due). No counterpart in the
+ * reference. */
+int64_t wt_batch_gate_offset(int ntunnels);
 /* Upload the descriptor table of a batch: grids[k] with params[k], current
  * buffer cur[k]. All tunnels share one storage form. If sums_dev is not NULL
  * (device array of 2*ntunnels*4 doubles) the reduced sums of tunnel k after a
@@ -187,8 +207,9 @@ int wt_batch_prepare(const wt_grid* grids, const wt_params* params, const int* c
 int wt_batch_encode(const wt_grid* grids, const wt_params* params, const int* cur, int ntunnels,
                     double* sums_dev, void* host_image, wt_batch_info* info);
 /* One step of every tunnel of the prepared batch; with parity p tunnel k
- * reads buffer cur[k]^p. flags as for wt_step; sums land in each grid's sums.
- * Kernel launches only: a caller may capture it in a graph. */
+ * reads buffer cur[k]^p. flags as for wt_step (WT_FLAG_NO_MAC subject to the
+ * gate bytes); sums land in each grid's sums. Kernel launches only: a caller
+ * may capture it in a graph. */
 int wt_batch_step(void* workspace, const wt_batch_info* info, int parity, int flags, void* stream);
 
 /* ---- particles, slices, surface pressure ---------------------------------- */

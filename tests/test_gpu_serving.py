@@ -13,19 +13,28 @@ def _requests():
                           max_sessions=8)
 
 
-def _alone(req):
+def _alone(req, slice_step=0):
+    """Per-step forces of the session run alone (every step storing its field),
+    and the serving loop's slice image after step slice_step."""
+    import torch
     from windtunnel import geometry
     from windtunnel._lib import WT_FLAG_FORCES
+    from windtunnel.session import FIELD_RANGES
     from windtunnel.tunnel import Tunnel
     mask, _ = geometry.ellipsoids(req.ellipsoids, req.dims, ground=req.ground)
     t = Tunnel(req.dims, req.form)
     t.init(req.uin)
     t.set_solid(mask)
-    out = []
-    for _ in range(req.steps):
+    out, img = [], None
+    nx, ny, nz = req.dims
+    for k in range(req.steps):
         t.step(1, req.uin, req.tau, req.smag, flags=WT_FLAG_FORCES)
         out.append(t.sums.cpu().numpy()[:3].copy())
-    return np.array(out)
+        if k + 1 == slice_step:
+            dev = torch.zeros(2 * nx * ny, dtype=torch.uint8, device='cuda')
+            t.slice(dev, 2, nz // 2, 0, req.uin, 40.0, *FIELD_RANGES[0])
+            img = dev.cpu().numpy()
+    return (np.array(out), img) if slice_step else np.array(out)
 
 
 @pytest.mark.parametrize('graphs', [False, True])
@@ -46,9 +55,12 @@ def test_sessions_identical_alone_and_in_the_mix(gpu, graphs):
         lv = srv.finished[r.sid]
         assert lv.done_steps == r.steps and len(lv.forces) == r.steps
         got = np.array(lv.forces)
-        want = _alone(r)
+        assert lv.slices == r.steps // r.slice_every and lv.slices > 0
+        # the field is stored only on the steps whose slice is due (WT_FLAG_NO_MAC with
+        # gates): the last slice equals the one taken alone after the same step
+        want, img = _alone(r, lv.last_slice_step)
         assert np.array_equal(got.view(np.uint64), want.view(np.uint64)), r.sid
-        assert lv.slices == r.steps // r.slice_every
+        assert np.array_equal(lv.slice_host.numpy(), img), (r.sid, lv.last_slice_step)
 
 
 def test_preemption_keeps_results_exact(gpu):

@@ -77,6 +77,36 @@ def test_noalloc_step_rejects_tunnels_beyond_its_scratch():
     assert 'exceed' in _lib.last_error() and str(1 << 26) in _lib.last_error()
 
 
+def test_flags_and_batch_gates_without_gpu_calls():
+    """Unknown flag bits, and WT_FLAG_NO_MAC on the fused advection step (whose
+    field the next advection reads), are rejected before any device call; the
+    encoded batch table opens every tunnel's gate. The fake buffer addresses are
+    never dereferenced."""
+    from windtunnel import _lib
+    lib = _lib.load()
+    g = _lib.Grid(16, 8, 8, 0, 1, 0, (ctypes.c_void_p * 2)(0x1000, 0x2000), 0x3000, 0x4000,
+                  (ctypes.c_void_p * 2)(0x5000, 0x6000), 0x7000, 0x8000)
+    p = _lib.Params(0.05, 0.52, 0.16)
+    assert lib.wt_step(ctypes.byref(g), 0, 1, ctypes.byref(p), 8, None) < 0
+    assert 'flag' in _lib.last_error()
+    box = _lib.SeedBox(2.0, 4.0, 4.0, 2.0, 2.0)
+    v = ctypes.c_void_p
+    assert lib.wt_step_advect(ctypes.byref(g), 0, ctypes.byref(p), _lib.WT_FLAG_NO_MAC, v(0x9000), v(0xa000),
+                              v(0xb000), None, 16, 1.0, 0.05, 100.0, 1, ctypes.byref(box), None) < 0
+    assert 'WT_FLAG_NO_MAC' in _lib.last_error()
+    nt = 3
+    grids = (_lib.Grid * nt)(g, g, g)
+    prm = (_lib.Params * nt)(p, p, p)
+    cur = (ctypes.c_int32 * nt)(0, 1, 0)
+    nbytes, off = lib.wt_batch_workspace_bytes(nt), lib.wt_batch_gate_offset(nt)
+    assert nbytes == off + 8 and off > 0
+    img = (ctypes.c_uint8 * nbytes)()
+    info = _lib.BatchInfo()
+    assert lib.wt_batch_encode(grids, prm, cur, nt, None, img, ctypes.byref(info)) == 0
+    assert list(img[off:off + 8]) == [1, 1, 1, 0, 0, 0, 0, 0]
+    assert info.ntunnels == nt and info.total_tiles == nt * (16 * 8 * 8 // 256)
+
+
 @pytest.mark.parametrize('dims,form,policy', [
     ((128, 64, 64), 0, 2),      # 2 x 19 x 4 B x 0.52M = 80 MB: all cached
     ((160, 96, 96), 0, 2),      # 213.75 MiB, within the 224 MiB bound

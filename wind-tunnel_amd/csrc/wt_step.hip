@@ -593,7 +593,7 @@ __device__ __forceinline__ void step_cell_general(const StepDesc& d, const typen
   if (__any((c & CLS_NEAR) != 0)) stage_link_rows(rows, d.cls, d.n, idx0, nx, nxy);
 
   if ((c & CLS_CODE) != 0) {  // solid cell, lbm.py:81-83
-    st_mac<FORM, POL == 1>(d.mac, idx, 0.f, 0.f, 0.f, 1.f);
+    if (!(flags & WT_FLAG_NO_MAC)) st_mac<FORM, POL == 1>(d.mac, idx, 0.f, 0.f, 0.f, 1.f);
     shell_write<R>(shout, x, y, z, zero, zero, zero, one);
     return;
   }
@@ -627,7 +627,7 @@ __device__ __forceinline__ void step_cell_general(const StepDesc& d, const typen
 #pragma unroll
     for (int i = 0; i < Q; i++) v[i] = S::store(face_feq<R>(i, br, bux, buy, buz, bsq), i);
     store_all(v);
-    st_mac<FORM, POL == 1>(d.mac, idx, (float)bux, (float)buy, (float)buz, (float)br);
+    if (!(flags & WT_FLAG_NO_MAC)) st_mac<FORM, POL == 1>(d.mac, idx, (float)bux, (float)buy, (float)buz, (float)br);
     if (flags & WT_FLAG_MASS) acc.m += (double)br;
     return;
   }
@@ -682,7 +682,7 @@ __device__ __forceinline__ void step_cell_general(const StepDesc& d, const typen
 #pragma unroll
   for (int i = 0; i < Q; i++) st[i] = S::put(out[i]);
   store_all(st);
-  st_mac<FORM, POL == 1>(d.mac, idx, (float)ux, (float)uy, (float)uz, (float)rho);
+  if (!(flags & WT_FLAG_NO_MAC)) st_mac<FORM, POL == 1>(d.mac, idx, (float)ux, (float)uy, (float)uz, (float)rho);
   shell_write<R>(shout, x, y, z, ux, uy, uz, rho);
   if (flags & WT_FLAG_MASS) acc.m += (double)rho;
   if ((flags & WT_FLAG_FORCES) && (c & CLS_FORCE)) add_force<FORM>(staged_force_links(rows, idx0, nx, nxy), st, acc);
@@ -759,7 +759,7 @@ __device__ __forceinline__ void step_cell_fast(const StepDesc& d, const typename
   } else {
     global_stores();
   }
-  st_mac<FORM, POL == 1>(d.mac, idx, (float)ux, (float)uy, (float)uz, (float)rho);
+  if (!(flags & WT_FLAG_NO_MAC)) st_mac<FORM, POL == 1>(d.mac, idx, (float)ux, (float)uy, (float)uz, (float)rho);
   if (!xface) shell_write<R>(shout, x, y, z, ux, uy, uz, rho);
   if (flags & WT_FLAG_MASS) acc.m += (double)rho;
 }
@@ -901,8 +901,9 @@ __global__ __launch_bounds__(BLOCK) WT_STEP_ATTR void k_step_adv(StepDesc d, Adv
 // striding over the items, measured 16 % slower on 48 uneven tunnels: the loop
 // raised the kernel to 96 registers, 5 waves per SIMD instead of 7.)
 template <int FORM>
-__global__ __launch_bounds__(BLOCK) void k_step_batch(const StepDesc* __restrict__ ds, int nd, uint32_t nitems,
-                                                      int parity, int backwards, int flags) {
+__global__ __launch_bounds__(BLOCK) void k_step_batch(const StepDesc* __restrict__ ds, const uint8_t* __restrict__ gates,
+                                                      int nd, uint32_t nitems, int parity, int backwards,
+                                                      int flags) {
   const uint32_t item = backwards ? nitems - 1u - blockIdx.x : blockIdx.x;
   int lo = 0, hi = nd - 1;
   while (lo < hi) {
@@ -912,10 +913,12 @@ __global__ __launch_bounds__(BLOCK) void k_step_batch(const StepDesc* __restrict
   const StepDesc d = ds[lo];
   const int p = parity ^ (d.flags & 1);
   const uint32_t tile = item - d.tile_begin;
+  // with WT_FLAG_NO_MAC a tunnel whose gate is set still stores its macroscopic field
+  const int fl = ((flags & WT_FLAG_NO_MAC) && gates[lo]) ? (flags & ~WT_FLAG_NO_MAC) : flags;
   switch (d.cpt) {
-    case 2: run_tile<FORM, 2, 0>(d, p, tile, flags); break;
-    case 4: run_tile<FORM, 4, 0>(d, p, tile, flags); break;
-    default: run_tile<FORM, 1, 0>(d, p, tile, flags); break;
+    case 2: run_tile<FORM, 2, 0>(d, p, tile, fl); break;
+    case 4: run_tile<FORM, 4, 0>(d, p, tile, fl); break;
+    default: run_tile<FORM, 1, 0>(d, p, tile, fl); break;
   }
 }
 
@@ -1192,10 +1195,10 @@ void launch_step_adv_f(const StepDesc& d, const AdvectArgs& a, Shape sh, int fla
 }
 
 template <int FORM>
-void launch_batch_f(const StepDesc* dd, int nd, uint32_t nitems, int parity, int backwards, int flags,
-                    hipStream_t s) {
-  hipLaunchKernelGGL((k_step_batch<FORM>), dim3(nitems), dim3(BLOCK), 0, s, dd, nd, nitems, parity, backwards,
-                     flags);
+void launch_batch_f(const StepDesc* dd, const uint8_t* gates, int nd, uint32_t nitems, int parity, int backwards,
+                    int flags, hipStream_t s) {
+  hipLaunchKernelGGL((k_step_batch<FORM>), dim3(nitems), dim3(BLOCK), 0, s, dd, gates, nd, nitems, parity,
+                     backwards, flags);
 }
 
 // The FP32 step kernels are instantiated in their own translation unit,
@@ -1207,16 +1210,17 @@ void launch_batch_f(const StepDesc* dd, int nd, uint32_t nitems, int parity, int
 template void launch_step_f<WT_FP32>(const StepDesc& d, int parity, Shape sh, int flags, hipStream_t s);
 template void launch_step_adv_f<WT_FP32>(const StepDesc& d, const AdvectArgs& a, Shape sh, int flags,
                                         hipStream_t s);
-template void launch_batch_f<WT_FP32>(const StepDesc* dd, int nd, uint32_t nitems, int parity, int backwards,
-                                      int flags, hipStream_t s);
+template void launch_batch_f<WT_FP32>(const StepDesc* dd, const uint8_t* gates, int nd, uint32_t nitems,
+                                      int parity, int backwards, int flags, hipStream_t s);
 
 }  // namespace wt
 #else
 extern template void launch_step_f<WT_FP32>(const StepDesc& d, int parity, Shape sh, int flags, hipStream_t s);
 extern template void launch_step_adv_f<WT_FP32>(const StepDesc& d, const AdvectArgs& a, Shape sh, int flags,
                                                hipStream_t s);
-extern template void launch_batch_f<WT_FP32>(const StepDesc* dd, int nd, uint32_t nitems, int parity,
-                                             int backwards, int flags, hipStream_t s);
+extern template void launch_batch_f<WT_FP32>(const StepDesc* dd, const uint8_t* gates, int nd,
+                                             uint32_t nitems, int parity, int backwards, int flags,
+                                             hipStream_t s);
 
 static int64_t nslots_of(int64_t n) { return (n + SLOT - 1) / SLOT; }
 static int64_t nchunks_of(int64_t n) { return (nslots_of(n) + CHUNK - 1) / CHUNK; }
@@ -1413,6 +1417,7 @@ int wt_step(const wt_grid* g, int cur, int nsteps, const wt_params* p, int flags
   if (int e = check_grid(g, true)) return e;
   if (!p) return fail(-1, "null params");
   if (nsteps < 0) return fail(-1, "negative step count %d", nsteps);
+  if (flags & ~(WT_FLAG_FORCES | WT_FLAG_MASS | WT_FLAG_NO_MAC)) return fail(-1, "unknown flag bits 0x%x", flags);
   if ((flags & (WT_FLAG_FORCES | WT_FLAG_MASS)) && (!g->partials || !g->sums))
     return fail(-1, "reduction flags need partials and sums buffers");
   Shape sh;
@@ -1422,7 +1427,7 @@ int wt_step(const wt_grid* g, int cur, int nsteps, const wt_params* p, int flags
   StepDesc d = make_desc(g, cur, p, 0);
   if (sh.remap & 4) d.global_addr = 1;  // shape bit 2: 64-bit global addressing at any size
   for (int k = 0; k < nsteps; k++) {
-    const int f = (k == nsteps - 1) ? (flags & (WT_FLAG_FORCES | WT_FLAG_MASS)) : 0;
+    const int f = ((k == nsteps - 1) ? (flags & (WT_FLAG_FORCES | WT_FLAG_MASS)) : 0) | (flags & WT_FLAG_NO_MAC);
     // with shape bit 1 the sweep direction alternates with the buffer parity
     Shape shk = sh;
     shk.remap = (sh.remap & 1) | (((sh.remap & 2) && ((cur ^ k) & 1)) ? 2 : 0);
@@ -1456,6 +1461,9 @@ int wt_step_advect(const wt_grid* g, int cur, const wt_params* p, int flags, con
   if (!p) return fail(-1, "null params");
   if ((flags & (WT_FLAG_FORCES | WT_FLAG_MASS)) && (!g->partials || !g->sums))
     return fail(-1, "reduction flags need partials and sums buffers");
+  if (flags & ~(WT_FLAG_FORCES | WT_FLAG_MASS))
+    return fail(-1, "wt_step_advect takes WT_FLAG_FORCES and WT_FLAG_MASS only (flags 0x%x; WT_FLAG_NO_MAC "
+                    "would leave unwritten the field the next step's advection reads)", flags);
   if (num_p < 0 || num_p >= ((int64_t)1 << 31)) return fail(-1, "bad particle count %lld", (long long)num_p);
   if (num_p > 0 && (!pos || !spd || !box || !mac_prev)) return fail(-1, "null particle buffer, field or seed box");
   if (num_p > 0 && mac_prev == (const void*)g->mac)
@@ -1524,6 +1532,14 @@ int64_t wt_batch_workspace_bytes(int ntunnels) {
     fail(-1, "ntunnels must be positive");
     return -1;
   }
+  return wt_batch_gate_offset(ntunnels) + (((int64_t)ntunnels + 7) & ~(int64_t)7);
+}
+
+int64_t wt_batch_gate_offset(int ntunnels) {
+  if (ntunnels <= 0) {
+    fail(-1, "ntunnels must be positive");
+    return -1;
+  }
   return (int64_t)ntunnels * (int64_t)(sizeof(StepDesc) + sizeof(RedDesc));
 }
 
@@ -1535,6 +1551,7 @@ int wt_batch_encode(const wt_grid* grids, const wt_params* params, const int* cu
   if (form < WT_FP32 || form > WT_FP64) return fail(-1, "unknown storage form %d", form);
   StepDesc* ds = (StepDesc*)host_image;
   RedDesc* rs = (RedDesc*)((char*)host_image + sizeof(StepDesc) * ntunnels);
+  uint8_t* gates = (uint8_t*)host_image + wt_batch_gate_offset(ntunnels);
   uint64_t tile = 0;
   int64_t max_chunks = 0;
   for (int k = 0; k < ntunnels; k++) {
@@ -1559,7 +1576,9 @@ int wt_batch_encode(const wt_grid* grids, const wt_params* params, const int* cu
     else
       rs[k] = RedDesc{grids[k].partials, grids[k].sums, grids[k].sums, (uint32_t)nslots_of(d.n)};
     max_chunks = std::max<int64_t>(max_chunks, nchunks_of(d.n));
+    gates[k] = 1;  // every tunnel stores its macroscopic field until the caller says otherwise
   }
+  for (int64_t k = ntunnels; k < ((ntunnels + 7) & ~7); k++) gates[k] = 0;
   info->total_tiles = (int64_t)tile;
   info->max_chunks = (int32_t)max_chunks;
   info->form = form;
@@ -1590,20 +1609,22 @@ int wt_batch_step(void* workspace, const wt_batch_info* info, int parity, int fl
   const int form = info->form;
   if (form < WT_FP32 || form > WT_FP64) return fail(-1, "unknown storage form %d", form);
   if (info->tile_order & ~2) return fail(-1, "unknown batch tile order bits 0x%x", info->tile_order);
+  if (flags & ~(WT_FLAG_FORCES | WT_FLAG_MASS | WT_FLAG_NO_MAC)) return fail(-1, "unknown flag bits 0x%x", flags);
   hipStream_t s = (hipStream_t)stream;
   const int nt = info->ntunnels;
   const StepDesc* dd = (const StepDesc*)workspace;
   const RedDesc* rd = (const RedDesc*)((const char*)workspace + sizeof(StepDesc) * nt);
-  const int f = flags & (WT_FLAG_FORCES | WT_FLAG_MASS);
+  const uint8_t* gates = (const uint8_t*)workspace + wt_batch_gate_offset(nt);
+  const int f = flags & (WT_FLAG_FORCES | WT_FLAG_MASS | WT_FLAG_NO_MAC);
   const uint32_t nitems = (uint32_t)info->total_tiles;
   const int backwards = ((info->tile_order & 2) && (parity & 1)) ? 1 : 0;
   switch (form) {
-    case WT_FP32: launch_batch_f<WT_FP32>(dd, nt, nitems, parity, backwards, f, s); break;
-    case WT_FP16: launch_batch_f<WT_FP16>(dd, nt, nitems, parity, backwards, f, s); break;
-    case WT_BF16: launch_batch_f<WT_BF16>(dd, nt, nitems, parity, backwards, f, s); break;
-    case WT_FP64: launch_batch_f<WT_FP64>(dd, nt, nitems, parity, backwards, f, s); break;
+    case WT_FP32: launch_batch_f<WT_FP32>(dd, gates, nt, nitems, parity, backwards, f, s); break;
+    case WT_FP16: launch_batch_f<WT_FP16>(dd, gates, nt, nitems, parity, backwards, f, s); break;
+    case WT_BF16: launch_batch_f<WT_BF16>(dd, gates, nt, nitems, parity, backwards, f, s); break;
+    case WT_FP64: launch_batch_f<WT_FP64>(dd, gates, nt, nitems, parity, backwards, f, s); break;
   }
-  if (f) {
+  if (f & (WT_FLAG_FORCES | WT_FLAG_MASS)) {
     hipLaunchKernelGGL(k_reduce1, dim3((unsigned)info->max_chunks, (unsigned)nt), dim3(BLOCK), 0, s, rd);
     hipLaunchKernelGGL(k_reduce2, dim3((unsigned)nt), dim3(BLOCK), 0, s, rd, parity & 1);
   }

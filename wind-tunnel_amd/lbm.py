@@ -20,6 +20,9 @@ Differences from the reference, all deliberate:
   * Particles live in cell-ordered slots (re-sorted every `sort_every`
     advections), each slot carrying its particle id; frame() still returns
     them in id order and every particle's values are unchanged.
+  * step(n) stores the macroscopic field only in its last lattice step, the
+    one whose field the advection and frame() read (lbm.py:443-453); face
+    cells read their own snapshot, so no other result depends on it.
 """
 import os
 import sys
@@ -150,7 +153,11 @@ class Solver:
     def step(self, steps, p):
         steps = int(steps)
         if steps > 0:
-            self.T.step(steps, p['uIn'], p['tau'], p['smag'], flags=_lib.WT_FLAG_FORCES)
+            # only the last step's macroscopic field is read (by the advection with
+            # dt = steps and by frame()), so the steps before it do not store it
+            if steps > 1:
+                self.T.step(steps - 1, p['uIn'], p['tau'], p['smag'], flags=_lib.WT_FLAG_NO_MAC)
+            self.T.step(1, p['uIn'], p['tau'], p['smag'], flags=_lib.WT_FLAG_FORCES)
             self._forces_fresh = True
         self.advance_particles(steps, p)
 

@@ -18,7 +18,7 @@ LIB_PATH = os.environ.get('WT_LIB_PATH') or os.path.join(HERE, 'libwt.so')
 WT_FP32, WT_FP16, WT_BF16, WT_FP64 = 0, 1, 2, 3
 FORMS = {'fp32': WT_FP32, 'fp16': WT_FP16, 'bf16': WT_BF16, 'fp64': WT_FP64}
 FORM_NAMES = {v: k for k, v in FORMS.items()}
-WT_FLAG_FORCES, WT_FLAG_MASS = 1, 2
+WT_FLAG_FORCES, WT_FLAG_MASS, WT_FLAG_NO_MAC = 1, 2, 4
 
 c_i32, c_i64, c_u32, c_f32, c_vp = ctypes.c_int32, ctypes.c_int64, ctypes.c_uint32, ctypes.c_float, ctypes.c_void_p
 
@@ -65,6 +65,7 @@ SIGNATURES = {
     'wt_forces': (c_i32, [P(Grid), c_i32, c_vp]),
     'wt_read_sums': (c_i32, [P(Grid), P(ctypes.c_double), c_vp]),
     'wt_batch_workspace_bytes': (c_i64, [c_i32]),
+    'wt_batch_gate_offset': (c_i64, [c_i32]),
     'wt_batch_prepare': (c_i32, [P(Grid), P(Params), P(c_i32), c_i32, c_vp, c_vp, P(BatchInfo), c_vp]),
     'wt_batch_encode': (c_i32, [P(Grid), P(Params), P(c_i32), c_i32, c_vp, c_vp, P(BatchInfo)]),
     'wt_batch_step': (c_i32, [c_vp, P(BatchInfo), c_i32, c_i32, c_vp]),

@@ -31,6 +31,9 @@ class Batch:
         nt = len(self.tunnels)
         nbytes = check(self.lib.wt_batch_workspace_bytes(nt), 'wt_batch_workspace_bytes')
         self.workspace = torch.empty(nbytes, dtype=torch.uint8, device=self.tunnels[0].device)
+        off = check(self.lib.wt_batch_gate_offset(nt), 'wt_batch_gate_offset')
+        # gate byte per tunnel: under WT_FLAG_NO_MAC only gated tunnels store their field
+        self.gates = self.workspace[off:off + nt]
         # force x, y, z and mass of every tunnel after a step of parity 0 / 1
         self.sums = torch.zeros((2, nt, 4), dtype=torch.float64, device=self.tunnels[0].device)
         self.info = BatchInfo()
@@ -57,6 +60,17 @@ class Batch:
             self.workspace.copy_(image, non_blocking=True)
         self.parity = 0
         self.graph = None
+
+    def set_gates(self, store, stream=None):
+        """Gate bytes for the following steps (asynchronous, in stream order): under
+        WT_FLAG_NO_MAC tunnel k stores its macroscopic field only if store[k] is true.
+        prepare() resets every gate to store."""
+        if len(store) != len(self.tunnels):
+            raise ValueError(f'{len(store)} gates for {len(self.tunnels)} tunnels')
+        host = torch.tensor([1 if v else 0 for v in store], dtype=torch.uint8).pin_memory()
+        s = stream if stream is not None else torch.cuda.current_stream(self.workspace.device)
+        with torch.cuda.stream(s):
+            self.gates.copy_(host, non_blocking=True)
 
     @property
     def cells(self):

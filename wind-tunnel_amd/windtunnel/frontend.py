@@ -264,15 +264,17 @@ class MultiEngine(threading.Thread):
                 self.cell_steps += steps * s.solver.n
 
     def _step_batched(self, key, members, steps):
-        from ._lib import WT_FLAG_FORCES
+        from ._lib import WT_FLAG_FORCES, WT_FLAG_NO_MAC
         from .batch import Batch
         sig = tuple((s.sid, s.version, s.solver.T.cur) for s in members)
         b = self._batches.get(key)
         if b is None or b.sig != sig:
             b = Batch([s.solver.T for s in members],
                       [tuple(float(s.p[k]) for k in LATTICE_KEYS) for s in members])
+            b.set_gates([False] * len(members))     # WT_FLAG_NO_MAC steps store no field
         for k in range(steps):
-            b.step(flags=WT_FLAG_FORCES if k == steps - 1 else 0)
+            # as Solver.step: only the last step's macroscopic field is read
+            b.step(flags=WT_FLAG_FORCES if k == steps - 1 else WT_FLAG_NO_MAC)
         self.batched_launches += steps
         sums = b.last_sums()
         for i, s in enumerate(members):

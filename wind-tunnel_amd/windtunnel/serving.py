@@ -36,7 +36,7 @@ import numpy as np
 import torch
 
 from . import geometry
-from ._lib import WT_FLAG_FORCES
+from ._lib import WT_FLAG_FORCES, WT_FLAG_NO_MAC
 from .batch import Batch
 from .session import FIELD_RANGES
 from .tunnel import Tunnel, state_bytes
@@ -111,6 +111,7 @@ class Live:
     last_done: float = 0.0
     forces: list = dataclasses.field(default_factory=list)
     slices: int = 0
+    last_slice_step: int = 0    # the session step whose field the last slice shows
     slice_dev: object = None
     slice_host: object = None
     saved: object = None        # host checkpoint while evicted
@@ -260,8 +261,9 @@ class Server:
                 lives = active.get(form, [])
                 if lives:
                     bt = Batch([lv.tunnel for lv in lives], [(lv.req.uin, lv.req.tau, lv.req.smag) for lv in lives])
+                    bt.gate_state = [True] * len(lives)      # as encoded
                     if self.use_graphs:
-                        bt.capture(WT_FLAG_FORCES)
+                        bt.capture(WT_FLAG_FORCES | WT_FLAG_NO_MAC)
                     bt.ring = [torch.zeros((2, len(lives), 4), dtype=torch.float64).pin_memory() for _ in range(4)]
                     bt.ring_pos = 0
                     batches[form] = bt
@@ -278,10 +280,17 @@ class Server:
             results = [dict() for _ in range(nsteps)]
             members = {}
             for form, lives, b in groups:
+                # the macroscopic field is stored only by the sessions whose slice is due
+                # after this tick (WT_FLAG_NO_MAC with their gate bytes set)
+                due = [bool(lv.req.slice_every) and ((lv.issued_steps + nsteps) // lv.req.slice_every >
+                                                      lv.issued_steps // lv.req.slice_every) for lv in lives]
+                if due != b.gate_state:
+                    b.set_gates(due)
+                    b.gate_state = due
                 if two:
                     b.replay()
                 else:
-                    b.step(WT_FLAG_FORCES)
+                    b.step(WT_FLAG_FORCES | WT_FLAG_NO_MAC)
                 recording = now() >= self.record_after
                 for lv in lives:
                     lv.issued_steps += nsteps
@@ -304,6 +313,7 @@ class Server:
                         lv.tunnel.slice(lv.slice_dev, 2, nz // 2, 0, lv.req.uin, 40.0, vmin, vmax)
                         lv.slice_host.copy_(lv.slice_dev, non_blocking=True)
                         lv.slices += 1
+                        lv.last_slice_step = lv.issued_steps
             ev = torch.cuda.Event()
             ev.record()
             self.max_tick_units = max(self.max_tick_units, units)

@@ -15,7 +15,7 @@ cells and 2^20 particles (FUSED_FORMS below).
 import numpy as np
 import torch
 
-from ._lib import WT_FLAG_FORCES
+from ._lib import WT_FLAG_FORCES, WT_FLAG_NO_MAC
 from .coefficients import ForceAverage
 from .tunnel import Tunnel
 
@@ -160,6 +160,8 @@ class Session:
                     events[1].record(main)
             self.pending = True
         else:
+            if not self.num_p and not frame:
+                flags |= WT_FLAG_NO_MAC      # no particle or slice reads this step's field
             if events is not None:
                 events[0].record(main)
             self.T.step(1, self.uin, self.tau, self.smag, flags=flags, stream=main)

@@ -4,9 +4,10 @@ config4: 48 independent tunnels of uneven size drawn by seed from
 {64x32x32, 96x48x48, 128x64x64, 160x80x80}, each with a seeded sphere or
 ellipsoid obstacle, inlet speed U(0.03, 0.08), tau U(0.51, 0.60), FP32; all of
 them advance in ONE batched launch per step, with forces every step fetched
-in one copy.
+in one copy. Nothing in the workload reads the macroscopic field, so its steps
+do not store it unless --store-mac asks for it (as the reference's step does).
 
-    python -m windtunnel.workloads [--steps 1000] [--warmup 20] [--form fp32] [--no-graphs]
+    python -m windtunnel.workloads [--steps 1000] [--warmup 20] [--form fp32] [--no-graphs] [--store-mac]
 """
 import time
 
@@ -14,7 +15,7 @@ import numpy as np
 import torch
 
 from . import geometry
-from ._lib import WT_FLAG_FORCES
+from ._lib import WT_FLAG_FORCES, WT_FLAG_NO_MAC
 from .batch import Batch
 from .tunnel import Tunnel
 
@@ -49,18 +50,25 @@ def build_config4(form='fp32', seed=4, ntunnels=48):
     return tunnels, params
 
 
-def run_config4(steps=1000, warmup=20, form='fp32', graphs=False, seed=4):
+def run_config4(steps=1000, warmup=20, form='fp32', graphs=False, seed=4, store_mac=False):
+    """The workload asks for forces every step and nothing that reads the
+    macroscopic field, so by default no step stores it (WT_FLAG_NO_MAC, every gate
+    closed); store_mac=True stores it every step as the reference's step does."""
     tunnels, params = build_config4(form, seed)
     b = Batch(tunnels, params)
+    flags = WT_FLAG_FORCES
+    if not store_mac:
+        flags |= WT_FLAG_NO_MAC
+        b.set_gates([False] * len(tunnels))
     if graphs:
-        b.capture(WT_FLAG_FORCES)
+        b.capture(flags)
     host = torch.zeros((2, len(tunnels), 4), dtype=torch.float64).pin_memory()
 
     def advance(k):
         if graphs:
             b.replay()
             return 2
-        b.step(WT_FLAG_FORCES)
+        b.step(flags)
         return 1
 
     k = 0
@@ -78,8 +86,8 @@ def run_config4(steps=1000, warmup=20, form='fp32', graphs=False, seed=4):
     e1.record()
     torch.cuda.synchronize()
     wall = time.perf_counter() - h0
-    return {'tunnels': len(tunnels), 'cells': cells, 'steps': done, 'graphs': graphs, 'wall_s': wall,
-            'gpu_ms': e0.elapsed_time(e1), 'mlups': cells * done / wall / 1e6,
+    return {'tunnels': len(tunnels), 'cells': cells, 'steps': done, 'graphs': graphs, 'store_mac': store_mac,
+            'wall_s': wall, 'gpu_ms': e0.elapsed_time(e1), 'mlups': cells * done / wall / 1e6,
             'ms_per_step': wall * 1e3 / done}
 
 
@@ -91,8 +99,10 @@ def main(argv=None):
     ap.add_argument('--warmup', type=int, default=20)
     ap.add_argument('--form', default='fp32')
     ap.add_argument('--no-graphs', action='store_true')
+    ap.add_argument('--store-mac', action='store_true', help='store the macroscopic field every step')
     args = ap.parse_args(argv)
-    print(json.dumps(run_config4(args.steps, args.warmup, args.form, graphs=not args.no_graphs)), flush=True)
+    print(json.dumps(run_config4(args.steps, args.warmup, args.form, graphs=not args.no_graphs,
+                                 store_mac=args.store_mac)), flush=True)
     return 0
 
 

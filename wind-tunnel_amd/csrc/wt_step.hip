@@ -506,20 +506,26 @@ __device__ __forceinline__ void add_force(uint32_t fl, const typename Form<FORM>
 // Boundary density and velocity of a face cell: inlet values at x == 0, else the
 // previous step's values of the nearest interior cell with the wall and outlet
 // overrides.
+// m: the snapshot values of the nearest interior cell (unused at x == 0).
 template <typename R>
-__device__ __forceinline__ void face_state(const StepDesc& d, const Shell<R>& shin, int x, int y, int z,
-                                           R& br, R& bux, R& buy, R& buz) {
+__device__ __forceinline__ void face_state_of(const StepDesc& d, const R* m, int x, int y, int z,
+                                              R& br, R& bux, R& buy, R& buz) {
   const R zero = (R)0, one = (R)1;
   if (x == 0) {
     br = one; bux = (R)d.uin; buy = zero; buz = zero;
     return;
   }
-  const R* m = shell_read<R>(shin, x, y, z);
   br = clampv(m[3], (R)0.9, (R)1.1);
   bux = m[0]; buy = m[1]; buz = m[2];
   if (y == 0 || y == d.ny - 1) buy = zero;
   if (z == 0 || z == d.nz - 1) buz = zero;
   if (x == d.nx - 1) { br = one; bux = maxv(bux, zero); }
+}
+template <typename R>
+__device__ __forceinline__ void face_state(const StepDesc& d, const Shell<R>& shin, int x, int y, int z,
+                                           R& br, R& bux, R& buy, R& buz) {
+  if (x == 0) face_state_of<R>(d, nullptr, x, y, z, br, bux, buy, buz);
+  else face_state_of<R>(d, shell_read<R>(shin, x, y, z), x, y, z, br, bux, buy, buz);
 }
 
 // Equilibrium populations of a face cell (lbm.py:104-107).
@@ -592,11 +598,7 @@ __device__ __forceinline__ void step_cell_general(const StepDesc& d, const typen
   const uint32_t idx0 = idx - (threadIdx.x & 63);
   if (__any((c & CLS_NEAR) != 0)) stage_link_rows(rows, d.cls, d.n, idx0, nx, nxy);
 
-  if ((c & CLS_CODE) != 0) {  // solid cell, lbm.py:81-83
-    if (!(flags & WT_FLAG_NO_MAC)) st_mac<FORM, POL == 1>(d.mac, idx, 0.f, 0.f, 0.f, 1.f);
-    shell_write<R>(shout, x, y, z, zero, zero, zero, one);
-    return;
-  }
+  const bool solid = (c & CLS_CODE) != 0;  // solid cell, lbm.py:81-83
   constexpr uint32_t ES = (uint32_t)sizeof(E);
   const uint32_t n = d.n;
   const uint32_t bo = pc * ES;
@@ -619,73 +621,86 @@ __device__ __forceinline__ void step_cell_general(const StepDesc& d, const typen
       global_stores();
     }
   };
-  if (face) {  // equilibrium boundaries, lbm.py:84-109
-    R br, bux, buy, buz;
-    face_state<R>(d, shin, x, y, z, br, bux, buy, buz);
-    const R bsq = (R)1.5 * (bux * bux + buy * buy + buz * buz);
-    E v[Q];
-#pragma unroll
-    for (int i = 0; i < Q; i++) v[i] = S::store(face_feq<R>(i, br, bux, buy, buz, bsq), i);
-    store_all(v);
-    if (!(flags & WT_FLAG_NO_MAC)) st_mac<FORM, POL == 1>(d.mac, idx, (float)bux, (float)buy, (float)buz, (float)br);
-    if (flags & WT_FLAG_MASS) acc.m += (double)br;
-    return;
-  }
-
-  // pull streaming with bounce-back and the moving belt, lbm.py:110-124:
-  // population i comes from element pc + pshift(i) (the upstream neighbour), or
-  // from the cell's own opposite plane pc + ppos(opp(i)) * nx when that
-  // neighbour is a wall; the loads go out in ascending upstream-row order
-  Links lk{0u, 0u};
-  if (c & CLS_NEAR) lk = staged_links(rows, idx0, nx, nxy);
-  R f[Q];
-  auto global_loads = [&]() {
-#pragma unroll
-    for (int k = 0; k < Q; k++) {
-      const int i = LOAD_ORDER[k];
-      const bool b = (lk.bb >> i) & 1u;
-      const uint32_t a = pc + (b ? (uint32_t)ppos(opp(i)) * (uint32_t)nx : pshift(i, nx, nxy));
-      f[i] = S::pop(ld_stream(fin + a));
-    }
-  };
-  if constexpr (sizeof(E) <= 4) {
-    if (!d.global_addr) {
-      // per-lane offset bo + bias + (element shift) * ES: non-negative, and below
-      // Q * n * ES + bias for every element of the array (no 32-bit wrap)
-      const uint32_t bias = (uint32_t)Q * ((uint32_t)nxy + (uint32_t)nx) * ES;
-      const __amdgpu_buffer_rsrc_t rin = __builtin_amdgcn_make_buffer_rsrc(
-          (void*)((const char*)fin - bias), (short)0, (int)((uint32_t)Q * n * ES + bias), BUF_GFX9_WORD3);
+  // Face lanes fetch their snapshot values first and take their boundary state
+  // after the interior lanes have issued their loads and collided, as in the fast
+  // path below (the other order measured 4.3 % (FP32) and 8.4 % (FP16) slower on
+  // the 48-tunnel batch, whose small tunnels put face lanes in many general waves,
+  // and 0.4-0.6 % slower at 33.5M cells).
+  typedef R R4 __attribute__((ext_vector_type(4)));
+  R4 sm = __builtin_nondeterministic_value(R4{});
+  if (face && !solid && x != 0) sm = *reinterpret_cast<const R4*>(shell_read<R>(shin, x, y, z));
+  R out[Q], ux, uy, uz, rho;
+  if (!solid && !face) {
+    // pull streaming with bounce-back and the moving belt, lbm.py:110-124:
+    // population i comes from element pc + pshift(i) (the upstream neighbour), or
+    // from the cell's own opposite plane pc + ppos(opp(i)) * nx when that
+    // neighbour is a wall; the loads go out in ascending upstream-row order
+    Links lk{0u, 0u};
+    if (c & CLS_NEAR) lk = staged_links(rows, idx0, nx, nxy);
+    R f[Q];
+    auto global_loads = [&]() {
 #pragma unroll
       for (int k = 0; k < Q; k++) {
         const int i = LOAD_ORDER[k];
-        // branch-free select between the two wave-uniform offsets
-        const uint32_t pull = pshift(i, nx, nxy) * ES + bias;
-        const uint32_t wall = (uint32_t)ppos(opp(i)) * (uint32_t)nx * ES + bias;
-        const uint32_t m = 0u - ((lk.bb >> i) & 1u);
-        f[i] = S::pop(RawIO<E>::ld(rin, bo + (pull + (m & (wall - pull))), 0u, POL == 1 && cx(i) == 0));
+        const bool b = (lk.bb >> i) & 1u;
+        const uint32_t a = pc + (b ? (uint32_t)ppos(opp(i)) * (uint32_t)nx : pshift(i, nx, nxy));
+        f[i] = S::pop(ld_stream(fin + a));
+      }
+    };
+    if constexpr (sizeof(E) <= 4) {
+      if (!d.global_addr) {
+        // per-lane offset bo + bias + (element shift) * ES: non-negative, and below
+        // Q * n * ES + bias for every element of the array (no 32-bit wrap)
+        const uint32_t bias = (uint32_t)Q * ((uint32_t)nxy + (uint32_t)nx) * ES;
+        const __amdgpu_buffer_rsrc_t rin = __builtin_amdgcn_make_buffer_rsrc(
+            (void*)((const char*)fin - bias), (short)0, (int)((uint32_t)Q * n * ES + bias), BUF_GFX9_WORD3);
+#pragma unroll
+        for (int k = 0; k < Q; k++) {
+          const int i = LOAD_ORDER[k];
+          // branch-free select between the two wave-uniform offsets
+          const uint32_t pull = pshift(i, nx, nxy) * ES + bias;
+          const uint32_t wall = (uint32_t)ppos(opp(i)) * (uint32_t)nx * ES + bias;
+          const uint32_t m = 0u - ((lk.bb >> i) & 1u);
+          f[i] = S::pop(RawIO<E>::ld(rin, bo + (pull + (m & (wall - pull))), 0u, POL == 1 && cx(i) == 0));
+        }
+      } else {
+        global_loads();
       }
     } else {
       global_loads();
     }
-  } else {
-    global_loads();
-  }
-  if (lk.belt) {
-    const R uin = (R)d.uin;
+    if (lk.belt) {
+      const R uin = (R)d.uin;
 #pragma unroll
-    for (int i = 1; i < Q; i++)
-      if (cx(i) != 0 && ((lk.belt >> i) & 1u)) f[i] = f[i] + (R)(6.0 * wd(i) * cx(i)) * uin;
+      for (int i = 1; i < Q; i++)
+        if (cx(i) != 0 && ((lk.belt >> i) & 1u)) f[i] = f[i] + (R)(6.0 * wd(i) * cx(i)) * uin;
+    }
+    collide_form<FORM>(f, x, nx, (R)d.tau, (R)d.smag, out, ux, uy, uz, rho);
   }
-  R out[Q], ux, uy, uz, rho;
-  collide_form<FORM>(f, x, nx, (R)d.tau, (R)d.smag, out, ux, uy, uz, rho);
+  asm volatile("" : "+v"(sm));
+  if (!solid && face) {  // equilibrium boundaries, lbm.py:84-109
+    const R m[4] = {sm.x, sm.y, sm.z, sm.w};
+    face_state_of<R>(d, m, x, y, z, rho, ux, uy, uz);
+    const R bsq = (R)1.5 * (ux * ux + uy * uy + uz * uz);
+    // the deviation forms store g = f - w, as store() does
+#pragma unroll
+    for (int i = 0; i < Q; i++)
+      out[i] = S::dev ? face_feq<R>(i, rho, ux, uy, uz, bsq) - (R)wd(i) : face_feq<R>(i, rho, ux, uy, uz, bsq);
+  }
+  if (solid) {
+    if (!(flags & WT_FLAG_NO_MAC)) st_mac<FORM, POL == 1>(d.mac, idx, 0.f, 0.f, 0.f, 1.f);
+    shell_write<R>(shout, x, y, z, zero, zero, zero, one);
+    return;
+  }
   E st[Q];
 #pragma unroll
   for (int i = 0; i < Q; i++) st[i] = S::put(out[i]);
   store_all(st);
   if (!(flags & WT_FLAG_NO_MAC)) st_mac<FORM, POL == 1>(d.mac, idx, (float)ux, (float)uy, (float)uz, (float)rho);
-  shell_write<R>(shout, x, y, z, ux, uy, uz, rho);
+  if (!face) shell_write<R>(shout, x, y, z, ux, uy, uz, rho);
   if (flags & WT_FLAG_MASS) acc.m += (double)rho;
-  if ((flags & WT_FLAG_FORCES) && (c & CLS_FORCE)) add_force<FORM>(staged_force_links(rows, idx0, nx, nxy), st, acc);
+  if ((flags & WT_FLAG_FORCES) && !face && (c & CLS_FORCE))
+    add_force<FORM>(staged_force_links(rows, idx0, nx, nxy), st, acc);
 }
 
 // ---- one cell, fast path: the whole wave is interior fluid away from walls -----------
@@ -708,16 +723,17 @@ __device__ __forceinline__ void step_cell_fast(const StepDesc& d, const typename
   const uint32_t bo = pc * ES;
   const uint32_t bias = (uint32_t)Q * (nxy + nx) * ES;
   R out[Q], ux, uy, uz, rho;
-  if (xface) {
-    // inlet / outlet lanes of this wave (lbm.py:84-109)
-    face_state<R>(d, shin, x, y, z, rho, ux, uy, uz);
-    const R bsq = (R)1.5 * (ux * ux + uy * uy + uz * uz);
-    // in the deviation forms the common store below takes g = f - w: the same
-    // subtraction store() makes on the face path
-#pragma unroll
-    for (int i = 0; i < Q; i++)
-      out[i] = S::dev ? face_feq<R>(i, rho, ux, uy, uz, bsq) - (R)wd(i) : face_feq<R>(i, rho, ux, uy, uz, bsq);
-  } else {
+  // Inlet / outlet lanes (lbm.py:84-109) take their boundary state after the
+  // interior lanes' collision, with the outlet lanes' snapshot values fetched
+  // before the wave's population loads go out: the other order held those waves'
+  // loads back behind the snapshot load (measured: the 48-tunnel batch 0.8 % and
+  // 128x64x64 tunnels 2.4-3.6 % faster, 33.5M cells the same, bit-identical).
+  // One vector, left unset on the other lanes: a split or zero-filled copy would
+  // make the compiler wait for the load at the join.
+  typedef R R4 __attribute__((ext_vector_type(4)));
+  R4 sm = __builtin_nondeterministic_value(R4{});
+  if (xface && x != 0) sm = *reinterpret_cast<const R4*>(shell_read<R>(shin, x, y, z));
+  if (!xface) {
     R f[Q];
     auto global_loads = [&]() {
 #pragma unroll
@@ -742,6 +758,20 @@ __device__ __forceinline__ void step_cell_fast(const StepDesc& d, const typename
       global_loads();
     }
     collide_form<FORM>(f, x, nx, (R)d.tau, (R)d.smag, out, ux, uy, uz, rho);
+  }
+  // the boundary arithmetic may not move above this point (the compiler would
+  // otherwise hoist it next to the snapshot loads, and wait for them there)
+  asm volatile("" : "+v"(sm));
+  if (xface) {
+    // inlet / outlet lanes of this wave (lbm.py:84-109)
+    const R m[4] = {sm.x, sm.y, sm.z, sm.w};
+    face_state_of<R>(d, m, x, y, z, rho, ux, uy, uz);
+    const R bsq = (R)1.5 * (ux * ux + uy * uy + uz * uz);
+    // in the deviation forms the common store below takes g = f - w: the same
+    // subtraction store() makes on the face path
+#pragma unroll
+    for (int i = 0; i < Q; i++)
+      out[i] = S::dev ? face_feq<R>(i, rho, ux, uy, uz, bsq) - (R)wd(i) : face_feq<R>(i, rho, ux, uy, uz, bsq);
   }
   auto global_stores = [&]() {
 #pragma unroll
@@ -1201,26 +1231,23 @@ void launch_batch_f(const StepDesc* dd, const uint8_t* gates, int nd, uint32_t n
                      backwards, flags);
 }
 
-// The FP32 step kernels are instantiated in their own translation unit,
-// wt_step_fp32.hip (this file with WT_STEP_FP32_UNIT defined), which the build
-// compiles with the compiler's max-ILP scheduling: measured 1.7 % faster for
-// the FP32 steps at 33.5M cells (5 waves per SIMD instead of 7), and 8 %
-// slower for FP16, whose kernels keep the default scheduling here.
+// The FP32 single-tunnel step kernels are instantiated in their own translation
+// unit, wt_step_fp32.hip (this file with WT_STEP_FP32_UNIT defined), which the
+// build compiles with the compiler's max-ILP scheduling: measured 1.7 % faster
+// for the FP32 steps at 33.5M cells (5 waves per SIMD instead of 7), and 8 %
+// slower for FP16, whose kernels keep the default scheduling here. The batched
+// FP32 step stays in this unit: with max-ILP it takes 99 registers (4 waves per
+// SIMD), with the default scheduling 80 (6 waves), 4.3 % faster on 48 tunnels.
 #ifdef WT_STEP_FP32_UNIT
 template void launch_step_f<WT_FP32>(const StepDesc& d, int parity, Shape sh, int flags, hipStream_t s);
 template void launch_step_adv_f<WT_FP32>(const StepDesc& d, const AdvectArgs& a, Shape sh, int flags,
                                         hipStream_t s);
-template void launch_batch_f<WT_FP32>(const StepDesc* dd, const uint8_t* gates, int nd, uint32_t nitems,
-                                      int parity, int backwards, int flags, hipStream_t s);
 
 }  // namespace wt
 #else
 extern template void launch_step_f<WT_FP32>(const StepDesc& d, int parity, Shape sh, int flags, hipStream_t s);
 extern template void launch_step_adv_f<WT_FP32>(const StepDesc& d, const AdvectArgs& a, Shape sh, int flags,
                                                hipStream_t s);
-extern template void launch_batch_f<WT_FP32>(const StepDesc* dd, const uint8_t* gates, int nd,
-                                             uint32_t nitems, int parity, int backwards, int flags,
-                                             hipStream_t s);
 
 static int64_t nslots_of(int64_t n) { return (n + SLOT - 1) / SLOT; }
 static int64_t nchunks_of(int64_t n) { return (nslots_of(n) + CHUNK - 1) / CHUNK; }

@@ -1,4 +1,4 @@
-// The FP32 step kernels (k_step, k_step_adv, k_step_batch for FP32 storage) in
+// The FP32 single-tunnel step kernels (k_step, k_step_adv for FP32 storage) in
 // a translation unit of their own, so that the build can compile them with the
 // compiler's max-ILP scheduling (-mllvm -amdgpu-sched-strategy=max-ilp) while
 // every other kernel keeps the default. Everything else in wt_step.hip is left

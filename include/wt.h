@@ -172,14 +172,17 @@ int wt_read_sums(const wt_grid* g, double out[4], void* stream);
  * the serving loop (app.py:150-202 generalised to many sessions).
  * total_tiles: work items (workgroups) of one step. tile_order: bit 1 sweeps
  * the items backwards on odd parities (set to 2 by wt_batch_prepare; a caller
- * may clear it before wt_batch_step; results do not depend on it). */
+ * may clear it before wt_batch_step; results do not depend on it). max_cpt:
+ * the largest cells-per-thread of the batch's tunnels (1, 2 or 4), which picks
+ * the kernel variant; set by wt_batch_encode and not to be changed. */
 typedef struct wt_batch_info {
   int64_t total_tiles;
   int32_t max_chunks;
   int32_t form;
   int32_t ntunnels;
   int32_t tile_order;
-  int32_t reserved[2];
+  int32_t max_cpt;
+  int32_t reserved;
 } wt_batch_info;
 /* Device workspace bytes for a batch of ntunnels (descriptor tables and the
  * per-tunnel gate bytes). */

@@ -105,6 +105,18 @@ def test_flags_and_batch_gates_without_gpu_calls():
     assert lib.wt_batch_encode(grids, prm, cur, nt, None, img, ctypes.byref(info)) == 0
     assert list(img[off:off + 8]) == [1, 1, 1, 0, 0, 0, 0, 0]
     assert info.ntunnels == nt and info.total_tiles == nt * (16 * 8 * 8 // 256)
+    # the batch's largest cells-per-thread picks the kernel variant; its tiles follow each shape
+    assert info.max_cpt == 1
+    g2, g4 = _lib.Grid.from_buffer_copy(g), _lib.Grid.from_buffer_copy(g)
+    g2.cpt, g4.cpt = 2, 4
+    for mixed, want in (((g, g2, g), 2), ((g4, g, g2), 4)):
+        grids = (_lib.Grid * nt)(*mixed)
+        assert lib.wt_batch_encode(grids, prm, cur, nt, None, img, ctypes.byref(info)) == 0
+        assert info.max_cpt == want
+        assert info.total_tiles == sum(-(-16 * 8 * 8 // (256 * m.cpt)) for m in mixed)
+    info.max_cpt = 3
+    assert lib.wt_batch_step(ctypes.c_void_p(0x9000), ctypes.byref(info), 0, 0, None) < 0
+    assert 'cells per thread' in _lib.last_error()
 
 
 @pytest.mark.parametrize('dims,form,policy', [

@@ -925,15 +925,21 @@ __global__ __launch_bounds__(BLOCK) WT_STEP_ATTR void k_step_adv(StepDesc d, Adv
 
 // Batched step: one workgroup per work item, tile t of tunnel k for every tunnel
 // of the batch, tunnel k's tiles holding 256 * ds[k].cpt cells (the launch shape
-// of its size class, one unrolled code path per cells-per-thread value). With
+// of its size class, one unrolled code path per cells-per-thread value up to
+// MAXCPT, the batch's largest: the variant without the 2- and 4-cell paths
+// needs 64-65 registers instead of 80; FP16 on 48 tunnels 10.7 % faster). With
 // `backwards` the items are dispatched from the last to the first. Each cell is
 // computed exactly as by k_step. (A persistent form, a grid of one occupancy
 // striding over the items, measured 16 % slower on 48 uneven tunnels: the loop
 // raised the kernel to 96 registers, 5 waves per SIMD instead of 7.)
-template <int FORM>
-__global__ __launch_bounds__(BLOCK) void k_step_batch(const StepDesc* __restrict__ ds, const uint8_t* __restrict__ gates,
-                                                      int nd, uint32_t nitems, int parity, int backwards,
-                                                      int flags) {
+// Waves per SIMD the compiler fits the batched step to: 8 for the FP16/BF16
+// one-cell variant (64 registers, no spills; measured 1.6 % faster than its
+// natural 7), no constraint otherwise.
+template <int FORM, int MAXCPT> constexpr int batch_min_waves() { return Form<FORM>::dev && MAXCPT == 1 ? 8 : 1; }
+template <int FORM, int MAXCPT>
+__global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(batch_min_waves<FORM, MAXCPT>())))
+void k_step_batch(const StepDesc* __restrict__ ds, const uint8_t* __restrict__ gates, int nd, uint32_t nitems,
+                  int parity, int backwards, int flags) {
   const uint32_t item = backwards ? nitems - 1u - blockIdx.x : blockIdx.x;
   int lo = 0, hi = nd - 1;
   while (lo < hi) {
@@ -945,11 +951,13 @@ __global__ __launch_bounds__(BLOCK) void k_step_batch(const StepDesc* __restrict
   const uint32_t tile = item - d.tile_begin;
   // with WT_FLAG_NO_MAC a tunnel whose gate is set still stores its macroscopic field
   const int fl = ((flags & WT_FLAG_NO_MAC) && gates[lo]) ? (flags & ~WT_FLAG_NO_MAC) : flags;
-  switch (d.cpt) {
-    case 2: run_tile<FORM, 2, 0>(d, p, tile, fl); break;
-    case 4: run_tile<FORM, 4, 0>(d, p, tile, fl); break;
-    default: run_tile<FORM, 1, 0>(d, p, tile, fl); break;
+  if constexpr (MAXCPT >= 4) {
+    if (d.cpt == 4) { run_tile<FORM, 4, 0>(d, p, tile, fl); return; }
   }
+  if constexpr (MAXCPT >= 2) {
+    if (d.cpt == 2) { run_tile<FORM, 2, 0>(d, p, tile, fl); return; }
+  }
+  run_tile<FORM, 1, 0>(d, p, tile, fl);
 }
 
 #ifndef WT_STEP_FP32_UNIT  // kernels launched only from the main unit
@@ -1224,11 +1232,24 @@ void launch_step_adv_f(const StepDesc& d, const AdvectArgs& a, Shape sh, int fla
   }
 }
 
+// The batched FP32 step runs best at 6 waves per SIMD (48 tunnels: 7 waves 1.9 %
+// slower, 5 waves 0.2 % slower); 22 KiB of LDS per workgroup holds it there
+// whatever its register count. FP16/BF16 take their 7 waves (65 registers).
+constexpr size_t FP32_BATCH_PAD_LDS = 22 * 1024;
+template <int FORM> constexpr size_t batch_pad_lds() { return FORM == WT_FP32 ? FP32_BATCH_PAD_LDS : 0; }
 template <int FORM>
 void launch_batch_f(const StepDesc* dd, const uint8_t* gates, int nd, uint32_t nitems, int parity, int backwards,
-                    int flags, hipStream_t s) {
-  hipLaunchKernelGGL((k_step_batch<FORM>), dim3(nitems), dim3(BLOCK), 0, s, dd, gates, nd, nitems, parity,
-                     backwards, flags);
+                    int flags, int max_cpt, hipStream_t s) {
+  const size_t pad = batch_pad_lds<FORM>();
+  if (max_cpt >= 4)
+    hipLaunchKernelGGL((k_step_batch<FORM, 4>), dim3(nitems), dim3(BLOCK), pad, s, dd, gates, nd, nitems, parity,
+                       backwards, flags);
+  else if (max_cpt == 2)
+    hipLaunchKernelGGL((k_step_batch<FORM, 2>), dim3(nitems), dim3(BLOCK), pad, s, dd, gates, nd, nitems, parity,
+                       backwards, flags);
+  else
+    hipLaunchKernelGGL((k_step_batch<FORM, 1>), dim3(nitems), dim3(BLOCK), pad, s, dd, gates, nd, nitems, parity,
+                       backwards, flags);
 }
 
 // The FP32 single-tunnel step kernels are instantiated in their own translation
@@ -1581,6 +1602,7 @@ int wt_batch_encode(const wt_grid* grids, const wt_params* params, const int* cu
   uint8_t* gates = (uint8_t*)host_image + wt_batch_gate_offset(ntunnels);
   uint64_t tile = 0;
   int64_t max_chunks = 0;
+  int max_cpt = 1;
   for (int k = 0; k < ntunnels; k++) {
     if (int e = check_grid(&grids[k], true)) return e;
     if (grids[k].form != form)
@@ -1591,6 +1613,7 @@ int wt_batch_encode(const wt_grid* grids, const wt_params* params, const int* cu
     StepDesc d = make_desc(&grids[k], 0, &params[k], 0);
     if (sh.remap & 4) d.global_addr = 1;
     d.cpt = sh.cpt;
+    max_cpt = std::max(max_cpt, sh.cpt);
     d.tile_begin = (uint32_t)tile;
     d.ntiles = (d.n + BLOCK * sh.cpt - 1) / (BLOCK * sh.cpt);
     d.flags = cur[k] & 1;
@@ -1611,7 +1634,8 @@ int wt_batch_encode(const wt_grid* grids, const wt_params* params, const int* cu
   info->form = form;
   info->ntunnels = ntunnels;
   info->tile_order = 2;
-  info->reserved[0] = info->reserved[1] = 0;
+  info->max_cpt = max_cpt;
+  info->reserved = 0;
   return 0;
 }
 
@@ -1636,6 +1660,8 @@ int wt_batch_step(void* workspace, const wt_batch_info* info, int parity, int fl
   const int form = info->form;
   if (form < WT_FP32 || form > WT_FP64) return fail(-1, "unknown storage form %d", form);
   if (info->tile_order & ~2) return fail(-1, "unknown batch tile order bits 0x%x", info->tile_order);
+  if (info->max_cpt != 1 && info->max_cpt != 2 && info->max_cpt != 4)
+    return fail(-1, "batch info has cells per thread %d (not set by wt_batch_encode?)", info->max_cpt);
   if (flags & ~(WT_FLAG_FORCES | WT_FLAG_MASS | WT_FLAG_NO_MAC)) return fail(-1, "unknown flag bits 0x%x", flags);
   hipStream_t s = (hipStream_t)stream;
   const int nt = info->ntunnels;
@@ -1646,10 +1672,10 @@ int wt_batch_step(void* workspace, const wt_batch_info* info, int parity, int fl
   const uint32_t nitems = (uint32_t)info->total_tiles;
   const int backwards = ((info->tile_order & 2) && (parity & 1)) ? 1 : 0;
   switch (form) {
-    case WT_FP32: launch_batch_f<WT_FP32>(dd, gates, nt, nitems, parity, backwards, f, s); break;
-    case WT_FP16: launch_batch_f<WT_FP16>(dd, gates, nt, nitems, parity, backwards, f, s); break;
-    case WT_BF16: launch_batch_f<WT_BF16>(dd, gates, nt, nitems, parity, backwards, f, s); break;
-    case WT_FP64: launch_batch_f<WT_FP64>(dd, gates, nt, nitems, parity, backwards, f, s); break;
+    case WT_FP32: launch_batch_f<WT_FP32>(dd, gates, nt, nitems, parity, backwards, f, info->max_cpt, s); break;
+    case WT_FP16: launch_batch_f<WT_FP16>(dd, gates, nt, nitems, parity, backwards, f, info->max_cpt, s); break;
+    case WT_BF16: launch_batch_f<WT_BF16>(dd, gates, nt, nitems, parity, backwards, f, info->max_cpt, s); break;
+    case WT_FP64: launch_batch_f<WT_FP64>(dd, gates, nt, nitems, parity, backwards, f, info->max_cpt, s); break;
   }
   if (f & (WT_FLAG_FORCES | WT_FLAG_MASS)) {
     hipLaunchKernelGGL(k_reduce1, dim3((unsigned)info->max_chunks, (unsigned)nt), dim3(BLOCK), 0, s, rd);

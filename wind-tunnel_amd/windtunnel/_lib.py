@@ -43,7 +43,7 @@ class SeedBox(ctypes.Structure):
 class BatchInfo(ctypes.Structure):
     """wt_batch_info"""
     _fields_ = [('total_tiles', c_i64), ('max_chunks', c_i32), ('form', c_i32), ('ntunnels', c_i32),
-                ('tile_order', c_i32), ('reserved', c_i32 * 2)]
+                ('tile_order', c_i32), ('max_cpt', c_i32), ('reserved', c_i32)]
 
 
 P = ctypes.POINTER

@@ -124,6 +124,31 @@ def test_batch_with_a_large_tunnel_equals_alone(gpu):
             assert _same([t.sums.cpu().numpy()], [bs[j]]), (k, t.nx)
 
 
+@pytest.mark.parametrize('cpt', [None, 2, 4])
+def test_batch_runs_every_tunnel_at_the_batch_cells_per_thread(gpu, cpt):
+    """By default a batch runs every tunnel at the cells per thread tuned for
+    batches of its storage form (or at an explicit one), whatever the tunnels'
+    own launch shapes, which it leaves alone; results equal the tunnels alone."""
+    from windtunnel import autotune
+    from windtunnel._lib import WT_FLAG_FORCES
+    from windtunnel.batch import Batch
+    alone, mixed = _make('fp16'), _make('fp16')
+    for t in mixed:
+        t.shape = (2, 0)
+    b = Batch(mixed, PARAMS, cpt=cpt)
+    want = cpt or autotune.batch_cpt('fp16')
+    assert b.info.max_cpt == want
+    assert b.info.total_tiles == sum(-(-t.n // (256 * want)) for t in mixed)
+    assert all(t.shape == (2, 0) for t in mixed)
+    for _ in range(2):
+        b.step(WT_FLAG_FORCES)
+        for t, p in zip(alone, PARAMS):
+            t.step(1, *p, flags=WT_FLAG_FORCES)
+    torch.cuda.synchronize()
+    for ta, tm in zip(alone, mixed):
+        assert _same(_state(ta)[:2], _state(tm)[:2])
+
+
 def test_batch_mixed_launch_shapes_and_sweep_orders(gpu):
     """One launch per step with each tunnel's own cells per thread (and 64-bit
     addressing for one of them), sweeping forwards or alternating: every tunnel
@@ -136,8 +161,9 @@ def test_batch_mixed_launch_shapes_and_sweep_orders(gpu):
     for a, m, sh in zip(alone, mixed, shapes):
         a.shape = (1, 0)
         m.shape = sh
-    b = Batch(mixed, PARAMS)
+    b = Batch(mixed, PARAMS, own_shapes=True)
     assert b.info.total_tiles == sum(-(-t.n // (256 * t.shape[0])) for t in mixed)
+    assert b.info.max_cpt == 4
     for order in (2, 0):
         b.info.tile_order = order
         flags = WT_FLAG_FORCES | WT_FLAG_MASS

@@ -108,7 +108,7 @@ def _c4_compare(form, oform, shapes=None, steps=3):
     if shapes:
         for t in tunnels:
             t.shape = shapes[(t.nx, t.ny, t.nz)]
-    b = Batch(tunnels, params)
+    b = Batch(tunnels, params, own_shapes=bool(shapes))
     flags = WT_FLAG_FORCES | WT_FLAG_MASS
     for _ in range(steps):
         b.step(flags)

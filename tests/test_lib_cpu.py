@@ -268,3 +268,25 @@ def test_autotune_classes_follow_the_library_cache_policy(form):
         n = dims[0] * dims[1] * dims[2]
         assert autotune.size_class(n, form) == name
         assert lib.wt_cache_policy(*dims, fid) == autotune.cache_policy(form, n)
+
+
+def test_autotune_batch_table():
+    """The committed tuning table holds one cells-per-thread value per storage form
+    for batches, chosen from the timed candidates; the tuning batch spans the
+    serving mix's sizes and the 48-tunnel batch's."""
+    import json
+    from windtunnel import autotune
+    with open(autotune.TUNING_PATH) as fh:
+        table = json.load(fh)
+    for form in ('fp32', 'fp16', 'bf16', 'fp64'):
+        c = autotune.batch_cpt(form)
+        assert c in autotune.BATCH_CPTS and c == table['batch_cpt'][form]
+        ms = table['batch_timings'][form]
+        assert min(ms, key=ms.get) == str(c)
+    assert autotune.batch_cpt('no-such-form') == 1
+    dims = autotune.batch_dims()
+    assert len(dims) == autotune.BATCH_SERVING + autotune.BATCH_SMALL
+    big = [d[0] * d[1] * d[2] for d in dims[:autotune.BATCH_SERVING]]
+    assert all(d[0] == 2 * d[1] == 2 * d[2] and d[1] % 8 == 0 for d in dims[:autotune.BATCH_SERVING])
+    assert min(big) > 0.3e6 and max(big) < 40e6
+    assert all(d in autotune.BATCH_SMALL_DIMS for d in dims[autotune.BATCH_SERVING:])

@@ -9,7 +9,14 @@ library uses for the tunnel's size) the tuner times every shape on a
 representative tunnel and stores the fastest in tuning.json beside this file;
 tunnels look their shape up by size class when they are created.
 
-    python -m windtunnel.autotune [--forms fp32,fp16,bf16,fp64] [--steps 40]
+Batches are tuned apart (tune_batch): one cells-per-thread value for every
+tunnel of a batch of that storage form, timed on a batch that spans the size
+classes. The batched kernel is compiled per largest cells-per-thread of its
+batch, and its one-cell variant holds more waves per SIMD, so a batch does not
+take its members' single-tunnel shapes (an FP16 batch of 60 serving-mix tunnels
+measured 7 % faster at one cell per thread than with the L class's two).
+
+    python -m windtunnel.autotune [--forms fp32,fp16,bf16,fp64] [--steps 40] [--batch-only]
 """
 import argparse
 import json
@@ -28,6 +35,11 @@ SMALL_CELLS = 200_000
 ELEM_BYTES = {'fp32': 4, 'fp16': 2, 'bf16': 2, 'fp64': 8}
 SHAPES = [(1, 0), (1, 1), (1, 2), (1, 3), (2, 0), (2, 1), (2, 2), (4, 0), (4, 1)]
 DEFAULT = (1, 0)
+# the batch tune_batch times: tunnels as batches hold them, 16 of the serving mix's
+# sizes (log-uniform 0.5M-33M cells, 2m x m x m) and 16 of the 48-tunnel batch's
+BATCH_SERVING, BATCH_SMALL = 16, 16
+BATCH_SMALL_DIMS = [(64, 32, 32), (96, 48, 48), (128, 64, 64), (160, 80, 80)]
+BATCH_CPTS = (1, 2, 4)
 
 _table = None
 
@@ -48,21 +60,31 @@ def size_class(cells, form_name='fp32'):
     return 'S' if cells < SMALL_CELLS else 'M'
 
 
-def load():
+def _load_file():
     global _table
     if _table is None:
         try:
             with open(TUNING_PATH) as fh:
-                _table = json.load(fh).get('shapes', {})
+                _table = json.load(fh)
         except (OSError, ValueError):
             _table = {}
     return _table
+
+
+def load():
+    """The single-tunnel shapes: form -> size class -> (cells per thread, tile order)."""
+    return _load_file().get('shapes', {})
 
 
 def shape_for(form_name, cells):
     t = load().get(form_name, {})
     v = t.get(size_class(cells, form_name))
     return tuple(v) if v else DEFAULT
+
+
+def batch_cpt(form_name):
+    """Cells per thread of every tunnel in a batch of this storage form."""
+    return int(_load_file().get('batch_cpt', {}).get(form_name, 1))
 
 
 def tune(forms, steps=40):
@@ -98,13 +120,89 @@ def tune(forms, steps=40):
     return out
 
 
+def _class_mask(dims):
+    from . import geometry
+    ells = geometry.car_body(dims, 7, count=8,
+                             box=((0.2 * dims[0], 0.5 * dims[0]), (1, 0.3 * dims[1]),
+                                  (0.35 * dims[2], 0.65 * dims[2])),
+                             axes=(0.03 * dims[1], 0.12 * dims[1]))
+    return geometry.ellipsoids(ells, dims, ground=True)[0]
+
+
+def batch_dims(seed=3):
+    """Grid sizes of the tuning batch (BATCH_SERVING + BATCH_SMALL tunnels)."""
+    import numpy as np
+    rng = np.random.default_rng(seed)
+    out = []
+    for _ in range(BATCH_SERVING):
+        cells = float(np.exp(rng.uniform(np.log(0.5e6), np.log(33e6))))
+        m = max(16, int(round((cells / 2) ** (1 / 3) / 8)) * 8)
+        out.append((2 * m, m, m))
+    out += [BATCH_SMALL_DIMS[int(k)] for k in rng.integers(len(BATCH_SMALL_DIMS), size=BATCH_SMALL)]
+    return out
+
+
+def tune_batch(forms, steps=20):
+    """Time one batch per storage form (batch_dims(): tunnels of the sizes batches
+    hold, forces every step, no macroscopic stores) at each uniform
+    cells-per-thread value; returns {'batch_cpt': {form: best},
+    'batch_timings': {form: {cpt: ms}}}."""
+    import numpy as np
+    import torch
+    from .batch import Batch
+    from .tunnel import Tunnel
+    out = {'batch_cpt': {}, 'batch_timings': {}}
+    for form in forms:
+        tunnels = []
+        for dims in batch_dims():
+            t = Tunnel(dims, form)
+            t.init(0.05)
+            t.set_solid(_class_mask(dims))
+            tunnels.append(t)
+        params = [(0.05, 0.52, 0.16)] * len(tunnels)
+        res = {}
+        for rep in range(2):
+            for cpt in (BATCH_CPTS if rep == 0 else BATCH_CPTS[::-1]):
+                b = Batch(tunnels, params, cpt=cpt)
+                b.set_gates([False] * len(tunnels))
+                for _ in range(3):
+                    b.step(1 | 4)
+                torch.cuda.synchronize()
+                e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+                e0.record()
+                for _ in range(steps):
+                    b.step(1 | 4)
+                e1.record()
+                torch.cuda.synchronize()
+                res.setdefault(str(cpt), []).append(e0.elapsed_time(e1) / steps)
+                del b
+        ms = {k: float(np.median(v)) for k, v in res.items()}
+        best = min(BATCH_CPTS, key=lambda c: ms[str(c)])
+        out['batch_cpt'][form] = best
+        out['batch_timings'][form] = ms
+        print(json.dumps({'form': form, 'batch_tunnels': len(tunnels), 'best_cpt': best, 'ms': ms}), flush=True)
+        del tunnels
+        torch.cuda.empty_cache()
+    return out
+
+
 def main(argv=None):
     ap = argparse.ArgumentParser()
     ap.add_argument('--forms', default='fp32,fp16,bf16,fp64')
     ap.add_argument('--steps', type=int, default=40)
     ap.add_argument('--out', default=TUNING_PATH)
+    ap.add_argument('--batch-only', action='store_true',
+                    help='re-tune only the batch entry, keeping the single-tunnel shapes of --out')
     args = ap.parse_args(argv)
-    res = tune(args.forms.split(','), args.steps)
+    forms = args.forms.split(',')
+    if args.batch_only:
+        with open(args.out) as fh:
+            res = json.load(fh)
+    else:
+        res = tune(forms, args.steps)
+    b = tune_batch(forms)
+    for key in ('batch_cpt', 'batch_timings'):
+        res.setdefault(key, {}).update(b[key])
     with open(args.out, 'w') as fh:
         json.dump(res, fh, indent=1)
     return 0

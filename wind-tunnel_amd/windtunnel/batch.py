@@ -1,6 +1,6 @@
 """Many resident tunnels advanced together: one launch of the batched step kernel
-per lattice step for all tunnels of one storage form, each tunnel keeping the
-launch shape (cells per thread) of its size class.
+per lattice step for all tunnels of one storage form, at the cells per thread
+tuned for batches (or, on request, each tunnel's own).
 
 The descriptor table (per-tunnel buffers, sizes, parameters, tile offsets) is
 encoded on the host once per batch composition (wt_batch_encode) and copied to
@@ -13,13 +13,17 @@ import ctypes
 
 import torch
 
-from . import _lib
+from . import _lib, autotune
 from ._lib import BatchInfo, Grid, Params, check, stream_handle
 
 
 class Batch:
-    def __init__(self, tunnels, params):
-        """tunnels: list of Tunnel (same storage form); params: list of (uin, tau, smag)."""
+    def __init__(self, tunnels, params, cpt=None, own_shapes=False):
+        """tunnels: list of Tunnel (same storage form); params: list of (uin, tau, smag).
+
+        Every tunnel runs at cpt cells per thread (default: the value tuned for
+        batches of this storage form, autotune.batch_cpt), unless own_shapes, when
+        each keeps the cells per thread of its own launch shape."""
         if not tunnels:
             raise ValueError('empty batch')
         forms = {t.form for t in tunnels}
@@ -27,6 +31,8 @@ class Batch:
             raise ValueError(f'a batch holds one storage form, got {sorted(forms)}')
         self.lib = _lib.load()
         self.tunnels = list(tunnels)
+        self.cpt = None if own_shapes else int(cpt if cpt is not None else
+                                                autotune.batch_cpt(_lib.FORM_NAMES[self.tunnels[0].form]))
         self.params = [tuple(float(v) for v in p) for p in params]
         nt = len(self.tunnels)
         nbytes = check(self.lib.wt_batch_workspace_bytes(nt), 'wt_batch_workspace_bytes')
@@ -43,9 +49,12 @@ class Batch:
 
     def prepare(self, stream=None):
         nt = len(self.tunnels)
-        # each tunnel's grid carries the launch shape of its size class: its tiles
-        # hold 256 * cpt cells inside the one launch
+        # each tunnel's tiles hold 256 * cpt cells inside the one launch: the batch's
+        # cells per thread, or with own_shapes that of the tunnel's launch shape
         grids = (Grid * nt)(*[t.grid for t in self.tunnels])
+        if self.cpt is not None:
+            for g in grids:
+                g.cpt = self.cpt
         prm = (Params * nt)(*[Params(*p) for p in self.params])
         cur = (ctypes.c_int32 * nt)(*[t.cur for t in self.tunnels])
         # the descriptor table is written into pinned host memory and copied in stream

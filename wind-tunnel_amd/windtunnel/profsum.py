@@ -3,9 +3,13 @@
     python -m windtunnel.profsum stats <rocprof out dir> [--top 15]
         per-kernel calls / average / total time from *_kernel_stats.csv
     python -m windtunnel.profsum traffic <fetch dir> <write dir> --copy-bytes B [--json-dir D]
+                                         [--copy-dirs FETCH_DIR WRITE_DIR] [--cells N] [--no-mac]
         HBM bytes per launch of each kernel from FETCH_SIZE and WRITE_SIZE
         (*_counter_collection.csv, KiB per dispatch), each scaled by what the
-        same counter reads for wt_stream_copy of B bytes (reads B, writes B)
+        same counter reads for wt_stream_copy of B bytes (reads B, writes B),
+        from the same runs or, with --copy-dirs, from other runs on the box;
+        batched steps (k_step_batch) are priced over --cells cells, without the
+        macroscopic store with --no-mac
 
 Prints markdown; with --json-dir also writes step_<form>_<dims>_traffic.json
 (k_step) and step_adv_<form>_<dims>_traffic.json (k_step_adv, the step fused
@@ -22,6 +26,7 @@ import re
 import sys
 
 BYTES_PER_CELL = {0: 169, 1: 85, 2: 85, 3: 321}
+MAC_BYTES = {0: 16, 1: 8, 2: 8, 3: 16}
 BYTES_PER_PARTICLE = 40   # id 4, position 16 read + 16 written, speed 4 written
 FORM_NAMES = {0: 'fp32', 1: 'fp16', 2: 'bf16', 3: 'fp64'}
 
@@ -61,11 +66,16 @@ def _per_kernel(rows, counter):
     return acc
 
 
-def traffic(fetch_dir, write_dir, copy_bytes, json_dir=None, cells=None, particles=0):
+def traffic(fetch_dir, write_dir, copy_bytes, json_dir=None, cells=None, particles=0, copy_dirs=None,
+            no_mac=False):
     fr = _per_kernel(_rows(fetch_dir, 'counter_collection.csv'), 'FETCH_SIZE')
     wr = _per_kernel(_rows(write_dir, 'counter_collection.csv'), 'WRITE_SIZE')
-    cf = [v for (k, _), vs in fr.items() if 'k_stream_copy' in k for v in vs]
-    cw = [v for (k, _), vs in wr.items() if 'k_stream_copy' in k for v in vs]
+    cfr, cwr = fr, wr
+    if copy_dirs:
+        cfr = _per_kernel(_rows(copy_dirs[0], 'counter_collection.csv'), 'FETCH_SIZE')
+        cwr = _per_kernel(_rows(copy_dirs[1], 'counter_collection.csv'), 'WRITE_SIZE')
+    cf = [v for (k, _), vs in cfr.items() if 'k_stream_copy' in k for v in vs]
+    cw = [v for (k, _), vs in cwr.items() if 'k_stream_copy' in k for v in vs]
     if not cf or not cw:
         raise SystemExit('no k_stream_copy dispatch in the counter runs')
     # median of the copy dispatches: KiB counted for copy_bytes read / written
@@ -85,8 +95,15 @@ def traffic(fetch_dir, write_dir, copy_bytes, json_dir=None, cells=None, particl
             continue
         fb = sum(vs) / len(vs) * f_scale
         wb = sum(ws) / len(ws) * w_scale
+        mb = re.search(r'k_step_batch<(\d)', name)
+        if mb:
+            form = int(mb.group(1))
+            algo = (BYTES_PER_CELL[form] - (MAC_BYTES[form] if no_mac else 0)) * cells
+            lines.append(f'| {short(name)} | {grid} | {len(vs)} | {fb / 1e6:.1f} | {wb / 1e6:.1f} | '
+                         f'{(fb + wb) / 1e6:.1f} | {algo / 1e6:.1f} | {(fb + wb) / algo:.4f} |')
+            continue
         m = re.search(r'k_step(_adv)?<(\d)', name)
-        if not m or 'batch' in name:
+        if not m:
             continue
         adv, form = bool(m.group(1)), int(m.group(2))
         algo = BYTES_PER_CELL[form] * cells + (BYTES_PER_PARTICLE * particles if adv else 0)
@@ -100,6 +117,7 @@ def traffic(fetch_dir, write_dir, copy_bytes, json_dir=None, cells=None, particl
                     'algorithmic_bytes_per_particle': BYTES_PER_PARTICLE if adv else 0,
                     'scale_fetch': f_scale, 'scale_write': w_scale}
     if json_dir:
+        os.makedirs(json_dir, exist_ok=True)
         for key, v in out.items():
             with open(os.path.join(json_dir, f'{key}_512x256x256_traffic.json'), 'w') as fh:
                 json.dump(v, fh, indent=1)
@@ -119,11 +137,14 @@ def main(argv=None):
     b.add_argument('--json-dir', default=None)
     b.add_argument('--cells', type=int, default=512 * 256 * 256)
     b.add_argument('--particles', type=int, default=0)
+    b.add_argument('--copy-dirs', nargs=2, default=None, metavar=('FETCH_DIR', 'WRITE_DIR'))
+    b.add_argument('--no-mac', action='store_true')
     args = ap.parse_args(argv)
     if args.cmd == 'stats':
         print(stats(args.dir, args.top))
     else:
-        print(traffic(args.fetch_dir, args.write_dir, args.copy_bytes, args.json_dir, args.cells, args.particles))
+        print(traffic(args.fetch_dir, args.write_dir, args.copy_bytes, args.json_dir, args.cells, args.particles,
+                      args.copy_dirs, args.no_mac))
     return 0
 
 

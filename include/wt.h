@@ -185,7 +185,8 @@ typedef struct wt_batch_info {
   int32_t reserved;
 } wt_batch_info;
 /* Device workspace bytes for a batch of ntunnels (descriptor tables and the
- * per-tunnel gate bytes). */
+ * per-tunnel gate bytes). The workspace passed to wt_batch_prepare and
+ * wt_batch_step must be 16-byte aligned (any device allocation is). */
 int64_t wt_batch_workspace_bytes(int ntunnels);
 /* Byte offset in the workspace of the batch's ntunnels gate bytes: with
  * WT_FLAG_NO_MAC, wt_batch_step stores the macroscopic field of tunnel k only

@@ -114,6 +114,8 @@ def test_flags_and_batch_gates_without_gpu_calls():
         assert lib.wt_batch_encode(grids, prm, cur, nt, None, img, ctypes.byref(info)) == 0
         assert info.max_cpt == want
         assert info.total_tiles == sum(-(-16 * 8 * 8 // (256 * m.cpt)) for m in mixed)
+    assert lib.wt_batch_step(ctypes.c_void_p(0x9004), ctypes.byref(info), 0, 0, None) < 0
+    assert 'aligned' in _lib.last_error()
     info.max_cpt = 3
     assert lib.wt_batch_step(ctypes.c_void_p(0x9000), ctypes.byref(info), 0, 0, None) < 0
     assert 'cells per thread' in _lib.last_error()

@@ -941,16 +941,33 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(batch_min
 void k_step_batch(const StepDesc* __restrict__ ds, const uint8_t* __restrict__ gates, int nd, uint32_t nitems,
                   int parity, int backwards, int flags) {
   const uint32_t item = backwards ? nitems - 1u - blockIdx.x : blockIdx.x;
-  int lo = 0, hi = nd - 1;
-  while (lo < hi) {
-    const int mid = (lo + hi + 1) >> 1;
-    if (ds[mid].tile_begin <= item) lo = mid; else hi = mid - 1;
+  // the item's tunnel: for FP16/BF16 the number of tunnels whose first tile is at
+  // or before it, less one (ds[0].tile_begin is 0), counted 64 tunnels per
+  // wave-wide vote: one round of loads instead of a chain of dependent ones
+  // (48 tunnels: 2.4 % faster); for FP32/FP64 a binary search, whose later start
+  // of each workgroup's streams measured faster there (the vote: 3.5 % slower)
+  int lo = 0;
+  if constexpr (Form<FORM>::dev) {
+    lo = -1;
+    for (int c0 = 0; c0 < nd; c0 += 64) {
+      const int k = c0 + (int)(threadIdx.x & 63);
+      lo += __popcll(__ballot(k < nd && ds[k].tile_begin <= item));
+    }
+  } else {
+    int hi = nd - 1;
+    while (lo < hi) {
+      const int mid = (lo + hi + 1) >> 1;
+      if (ds[mid].tile_begin <= item) lo = mid; else hi = mid - 1;
+    }
   }
   const StepDesc d = ds[lo];
   const int p = parity ^ (d.flags & 1);
   const uint32_t tile = item - d.tile_begin;
   // with WT_FLAG_NO_MAC a tunnel whose gate is set still stores its macroscopic field
-  const int fl = ((flags & WT_FLAG_NO_MAC) && gates[lo]) ? (flags & ~WT_FLAG_NO_MAC) : flags;
+  // (read as the gate's aligned word, a scalar load: the gate bytes are padded to a
+  // multiple of 8 and start 8-byte aligned in the workspace)
+  const uint32_t gw = reinterpret_cast<const uint32_t*>(gates)[lo >> 2];
+  const int fl = ((flags & WT_FLAG_NO_MAC) && ((gw >> (8 * (lo & 3))) & 0xffu)) ? (flags & ~WT_FLAG_NO_MAC) : flags;
   if constexpr (MAXCPT >= 4) {
     if (d.cpt == 4) { run_tile<FORM, 4, 0>(d, p, tile, fl); return; }
   }
@@ -1642,6 +1659,7 @@ int wt_batch_encode(const wt_grid* grids, const wt_params* params, const int* cu
 int wt_batch_prepare(const wt_grid* grids, const wt_params* params, const int* cur, int ntunnels,
                      double* sums_dev, void* workspace, wt_batch_info* info, void* stream) {
   if (!workspace || ntunnels <= 0) return fail(-1, "null workspace or empty batch");
+  if ((uintptr_t)workspace & 15) return fail(-1, "batch workspace is not 16-byte aligned");
   std::vector<char> image((size_t)wt_batch_workspace_bytes(ntunnels));
   if (int e = wt_batch_encode(grids, params, cur, ntunnels, sums_dev, image.data(), info)) return e;
   hipStream_t s = (hipStream_t)stream;
@@ -1657,6 +1675,7 @@ int wt_batch_prepare(const wt_grid* grids, const wt_params* params, const int* c
 // may capture it in a graph (two calls, p = 0 and p = 1, per replay).
 int wt_batch_step(void* workspace, const wt_batch_info* info, int parity, int flags, void* stream) {
   if (!workspace || !info || info->ntunnels <= 0 || info->total_tiles <= 0) return fail(-1, "empty batch");
+  if ((uintptr_t)workspace & 15) return fail(-1, "batch workspace is not 16-byte aligned");
   const int form = info->form;
   if (form < WT_FP32 || form > WT_FP64) return fail(-1, "unknown storage form %d", form);
   if (info->tile_order & ~2) return fail(-1, "unknown batch tile order bits 0x%x", info->tile_order);

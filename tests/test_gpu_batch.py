@@ -176,3 +176,42 @@ def test_batch_mixed_launch_shapes_and_sweep_orders(gpu):
             for j, (ta, tm) in enumerate(zip(alone, mixed)):
                 assert _same(_state(ta)[:2], _state(tm)[:2]), (order, k, j)
                 assert _same([ta.sums.cpu().numpy()], [bs[j]]), (order, k, j)
+
+
+@pytest.mark.parametrize('form', ['fp16', 'fp32'])
+def test_batch_of_more_than_64_tunnels_equals_alone(gpu, form):
+    """70 tunnels of four sizes in one batch: the FP16/BF16 kernel finds a work
+    item's tunnel by wave-wide votes over 64 descriptors at a time, so this batch
+    needs two of them (FP32 searches); every tunnel ends bit-identical to itself
+    run alone, and its FP64 sums equal the alone sums."""
+    from windtunnel._lib import WT_FLAG_FORCES, WT_FLAG_MASS
+    from windtunnel.batch import Batch
+    from windtunnel.tunnel import Tunnel
+    sizes = [(16, 8, 8), (24, 12, 10), (40, 8, 12), (8, 8, 8)]
+    rng = np.random.default_rng(70)
+    alone, mixed, params = [], [], []
+    for k in range(70):
+        dims = sizes[k % len(sizes)]
+        p = (float(rng.uniform(0.03, 0.08)), float(rng.uniform(0.51, 0.6)), 0.16)
+        mask = _mask(dims, k) if dims[1] >= 12 else np.zeros(dims[0] * dims[1] * dims[2], np.uint8)
+        pair = []
+        for _ in range(2):
+            t = Tunnel(dims, form)
+            t.init(p[0])
+            t.set_solid(mask)
+            pair.append(t)
+        alone.append(pair[0])
+        mixed.append(pair[1])
+        params.append(p)
+    b = Batch(mixed, params)
+    assert b.info.ntunnels == 70
+    flags = WT_FLAG_FORCES | WT_FLAG_MASS
+    for _ in range(3):
+        b.step(flags)
+        for t, p in zip(alone, params):
+            t.step(1, *p, flags=flags)
+    torch.cuda.synchronize()
+    bs = b.last_sums().cpu().numpy()
+    for j, (ta, tm) in enumerate(zip(alone, mixed)):
+        assert _same(_state(ta)[:2], _state(tm)[:2]), j
+        assert _same([ta.sums.cpu().numpy()], [bs[j]]), j

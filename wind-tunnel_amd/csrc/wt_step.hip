@@ -931,7 +931,9 @@ __global__ __launch_bounds__(BLOCK) WT_STEP_ATTR void k_step_adv(StepDesc d, Adv
 // `backwards` the items are dispatched from the last to the first. Each cell is
 // computed exactly as by k_step. (A persistent form, a grid of one occupancy
 // striding over the items, measured 16 % slower on 48 uneven tunnels: the loop
-// raised the kernel to 96 registers, 5 waves per SIMD instead of 7.)
+// raised the kernel to 96 registers, 5 waves per SIMD instead of 7; on the
+// one-cell variant it spilled 21-74 scalar registers, 106-115 VGPRs for FP32,
+// 25 VGPRs to scratch for FP16.)
 // Waves per SIMD the compiler fits the batched step to: 8 for the FP16/BF16
 // one-cell variant (64 registers, no spills; measured 1.6 % faster than its
 // natural 7), no constraint otherwise.

@@ -70,7 +70,10 @@ extern "C" {
  *             bit 2 64-bit global population addresses even where 32-bit
  *             buffer offsets would do (the form large tunnels take by
  *             themselves). Every call reads the shape from the grid it is
- *             given: the library keeps no launch state between calls. */
+ *             given: the library keeps no launch state between calls.
+ * Alignment:  f, cls, mac and shell 16-byte aligned (the FP64 shell 32-byte),
+ *             partials and sums 8-byte, as every device allocation is; calls
+ *             reject grids with other buffers. */
 typedef struct wt_grid {
   int32_t nx, ny, nz;
   int32_t form;

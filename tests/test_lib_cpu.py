@@ -65,6 +65,34 @@ def test_launch_shape_is_checked_per_grid_before_any_launch():
     del fake
 
 
+@pytest.mark.parametrize('form,field,addr', [(0, 'mac', 0x4008), (1, 'cls', 0x3004), (0, 'f1', 0x2002),
+                                             (3, 'shell0', 0x5010), (0, 'partials', 0x7004)])
+def test_misaligned_grid_buffers_are_rejected_before_any_launch(form, field, addr):
+    """Grid buffers are 16-byte aligned (the FP64 face snapshot 32, partials and
+    sums 8): a misaligned one is an error, not a kernel reading across it (the fake
+    buffer addresses are never dereferenced)."""
+    from windtunnel import _lib
+    lib = _lib.load()
+    f = [0x1000, 0x2000]
+    sh = [0x5000, 0x6000]
+    cls, mac, part = 0x3000, 0x4000, 0x7000
+    if field == 'mac':
+        mac = addr
+    elif field == 'cls':
+        cls = addr
+    elif field == 'f1':
+        f[1] = addr
+    elif field == 'shell0':
+        sh[0] = addr
+    else:
+        part = addr
+    g = _lib.Grid(16, 8, 8, form, 1, 0, (ctypes.c_void_p * 2)(*f), cls, mac, (ctypes.c_void_p * 2)(*sh), part,
+                  0x8000)
+    p = _lib.Params(0.05, 0.52, 0.16)
+    assert lib.wt_step(ctypes.byref(g), 0, 1, ctypes.byref(p), 0, None) < 0
+    assert 'aligned' in _lib.last_error()
+
+
 def test_noalloc_step_rejects_tunnels_beyond_its_scratch():
     """wt_step_noalloc serves tunnels of at most 2^26 cells and says so, before any
     device call (the fake buffer addresses are never dereferenced)."""

@@ -1327,6 +1327,16 @@ int check_grid(const wt_grid* g, bool need_state) {
   if (g->form < WT_FP32 || g->form > WT_FP64) return fail(-1, "unknown storage form %d", g->form);
   if (need_state && (!g->f[0] || !g->f[1] || !g->cls || !g->mac || !g->shell[0] || !g->shell[1]))
     return fail(-1, "grid has a null device buffer");
+  // the kernels read the face snapshot four reals at a time, store the macroscopic
+  // field as one vector per cell and the populations and class bytes through
+  // dword-addressed buffers: every state buffer 16-byte aligned (the FP64
+  // snapshot 32), as any device allocation is
+  if (need_state) {
+    const uintptr_t a = (uintptr_t)g->f[0] | (uintptr_t)g->f[1] | (uintptr_t)g->cls | (uintptr_t)g->mac;
+    const uintptr_t sh = (uintptr_t)g->shell[0] | (uintptr_t)g->shell[1];
+    if ((a & 15) || (sh & (g->form == WT_FP64 ? 31 : 15)) || ((uintptr_t)g->partials & 7) || ((uintptr_t)g->sums & 7))
+      return fail(-1, "grid buffers must be 16-byte aligned (the FP64 face snapshot 32-byte)");
+  }
   return 0;
 }
 

@@ -227,6 +227,11 @@ def main():
         c4m = run_config4(steps=args.c4_steps, warmup=20, graphs=True, store_mac=True)
         log(json.dumps(c4m))
         c4['mac_every_step'] = c4m
+        # the same steps by the persistent batched kernel (one grid of full occupancy
+        # claiming the 48 tunnels' tiles in order)
+        c4p = run_config4(steps=args.c4_steps, warmup=20, graphs=True, persistent=True)
+        log(json.dumps(c4p))
+        c4['persistent_kernel'] = c4p
         torch.cuda.empty_cache()
     serving = None
     if args.serve_seconds > 0:
@@ -299,7 +304,8 @@ def main():
                                  'cells': c4['cells'], 'steps': c4['steps'], 'ms_per_step': round(c4['ms_per_step'], 4),
                                  'launch': 'one batched launch per step (graph replay of 2 steps), forces every step',
                                  'macroscopic_field': 'not stored: nothing in the workload reads it (WT_FLAG_NO_MAC)',
-                                 'value_storing_field_every_step': round(c4['mac_every_step']['mlups'], 1)}
+                                 'value_storing_field_every_step': round(c4['mac_every_step']['mlups'], 1),
+                                 'value_persistent_kernel': round(c4['persistent_kernel']['mlups'], 1)}
     if serving:
         line['serving'] = {'step_latency_ms_p50': round(serving['step_latency_ms']['p50'], 3),
                            'step_latency_ms_p99': round(serving['step_latency_ms']['p99'], 3),

@@ -40,6 +40,9 @@ extern "C" {
  * of its steps: lbm.py:443-453). In a batch, tunnels whose gate byte is set
  * (wt_batch_gate_offset) still store it. No counterpart in the reference. */
 #define WT_FLAG_NO_MAC 4
+/* wt_batch_info.tile_order bits */
+#define WT_BATCH_BACKWARDS 2
+#define WT_BATCH_PERSISTENT 4
 
 /* One tunnel's device state. n = nx*ny*nz; sizes come from wt_buffer_sizes().
  * f[0], f[1]: ping-pong populations, row-blocked SoA: each x-row
@@ -173,9 +176,14 @@ int wt_read_sums(const wt_grid* g, double out[4], void* stream);
  * 256 * grids[t].cpt cells (each tunnel keeps the launch shape of its size
  * class). Replaces running one Solver.step per tunnel (lbm.py:442-447) for
  * the serving loop (app.py:150-202 generalised to many sessions).
- * total_tiles: work items (workgroups) of one step. tile_order: bit 1 sweeps
- * the items backwards on odd parities (set to 2 by wt_batch_prepare; a caller
- * may clear it before wt_batch_step; results do not depend on it). max_cpt:
+ * total_tiles: work items of one step. tile_order: bit 1 (WT_BATCH_BACKWARDS)
+ * sweeps the items backwards on odd parities (set by wt_batch_prepare; a caller
+ * may clear it before wt_batch_step); bit 2 (WT_BATCH_PERSISTENT, set by the
+ * caller) runs the persistent kernel: one grid of the device's full occupancy
+ * whose workgroups stride over the items, instead of one workgroup per item.
+ * Results do not depend on either bit. persist_blocks: workgroups of a
+ * persistent launch (0, the default: the device's full occupancy; a caller may
+ * set fewer). max_cpt:
  * the largest cells-per-thread of the batch's tunnels (1, 2 or 4), which picks
  * the kernel variant; set by wt_batch_encode and not to be changed. */
 typedef struct wt_batch_info {
@@ -185,7 +193,7 @@ typedef struct wt_batch_info {
   int32_t ntunnels;
   int32_t tile_order;
   int32_t max_cpt;
-  int32_t reserved;
+  int32_t persist_blocks;
 } wt_batch_info;
 /* Device workspace bytes for a batch of ntunnels (descriptor tables and the
  * per-tunnel gate bytes). The workspace passed to wt_batch_prepare and

@@ -1,6 +1,9 @@
 """Batched tunnels: every tunnel advanced inside one batched launch must equal
 the same tunnel advanced alone, bit for bit (populations, macroscopic field and
-the FP64 force/mass sums), also when replayed from a captured graph."""
+the FP64 force/mass sums), also when replayed from a captured graph; for both
+batched kernels: one workgroup per item, and the persistent one (PERSIST: run at
+the device's full occupancy, -1, or with a few workgroups striding over many
+items each)."""
 import numpy as np
 import pytest
 import torch
@@ -39,18 +42,25 @@ def _state(t):
     return (t.raw_populations().cpu().numpy().copy(), t.mac.cpu().numpy().copy(), t.sums.cpu().numpy().copy())
 
 
+# batched kernel of a test case: 0 one workgroup per item, -1 persistent at full
+# occupancy, k > 0 persistent with k workgroups
+def _batch(tunnels, params, persist=0, **kw):
+    from windtunnel.batch import Batch
+    return Batch(tunnels, params, persistent=persist != 0, persist_blocks=max(persist, 0), **kw)
+
+
 def _same(a, b):
     return all(np.array_equal(np.ascontiguousarray(x).view(np.uint8), np.ascontiguousarray(y).view(np.uint8))
                for x, y in zip(a, b))
 
 
+@pytest.mark.parametrize('persist', [0, -1, 7])
 @pytest.mark.parametrize('form', ['fp32', 'fp16', 'bf16', 'fp64'])
-def test_batch_equals_alone(gpu, form):
+def test_batch_equals_alone(gpu, form, persist):
     from windtunnel._lib import WT_FLAG_FORCES, WT_FLAG_MASS
-    from windtunnel.batch import Batch
     alone = _make(form, pre_steps=(2,))
     mixed = _make(form, pre_steps=(2,))
-    b = Batch(mixed, PARAMS)
+    b = _batch(mixed, PARAMS, persist)
     flags = WT_FLAG_FORCES | WT_FLAG_MASS
     for k in range(9):
         b.step(flags)
@@ -64,13 +74,13 @@ def test_batch_equals_alone(gpu, form):
             assert _same([ta.sums.cpu().numpy()], [bs[j]]), (form, k, ta.nx)
 
 
-def test_batch_graph_replay_equals_eager(gpu):
+@pytest.mark.parametrize('persist', [0, 5])
+def test_batch_graph_replay_equals_eager(gpu, persist):
     from windtunnel._lib import WT_FLAG_FORCES
-    from windtunnel.batch import Batch
     eager = _make('fp32')
     graphed = _make('fp32')
-    be = Batch(eager, PARAMS)
-    bg = Batch(graphed, PARAMS)
+    be = _batch(eager, PARAMS)
+    bg = _batch(graphed, PARAMS, persist)
     bg.capture(WT_FLAG_FORCES)
     for rep in range(5):
         be.step(WT_FLAG_FORCES)
@@ -92,13 +102,14 @@ def test_batch_rejects_mixed_forms(gpu):
         Batch([Tunnel((8, 8, 8), 'fp32'), Tunnel((8, 8, 8), 'fp16')], [(0.05, 0.52, 0.16)] * 2)
 
 
-def test_batch_with_a_large_tunnel_equals_alone(gpu):
+@pytest.mark.parametrize('persist', [0, -1])
+def test_batch_with_a_large_tunnel_equals_alone(gpu, persist):
     """A 2.4M-cell tunnel (10 first-level reduction chunks) batched with three
     small ones (1 chunk each): every tunnel's force/mass sums equal the same
-    tunnel stepped alone, bit for bit."""
+    tunnel stepped alone, bit for bit. Its 9.4k items outnumber the persistent
+    kernel's workgroups at full occupancy."""
     from windtunnel import geometry
     from windtunnel._lib import WT_FLAG_FORCES, WT_FLAG_MASS
-    from windtunnel.batch import Batch
     from windtunnel.tunnel import Tunnel
     big_dims, big_p = (192, 112, 112), (0.05, 0.52, 0.16)
     assert big_dims[0] * big_dims[1] * big_dims[2] > 8 * 1024 * 256
@@ -112,7 +123,7 @@ def test_batch_with_a_large_tunnel_equals_alone(gpu):
     alone = _make('fp32')[:3] + [big()]
     mixed = _make('fp32')[:3] + [big()]
     params = PARAMS[:3] + [big_p]
-    b = Batch(mixed, params)
+    b = _batch(mixed, params, persist)
     flags = WT_FLAG_FORCES | WT_FLAG_MASS
     for k in range(4):
         b.step(flags)
@@ -149,23 +160,23 @@ def test_batch_runs_every_tunnel_at_the_batch_cells_per_thread(gpu, cpt):
         assert _same(_state(ta)[:2], _state(tm)[:2])
 
 
-def test_batch_mixed_launch_shapes_and_sweep_orders(gpu):
+@pytest.mark.parametrize('persist', [0, 3])
+def test_batch_mixed_launch_shapes_and_sweep_orders(gpu, persist):
     """One launch per step with each tunnel's own cells per thread (and 64-bit
     addressing for one of them), sweeping forwards or alternating: every tunnel
     gets the bits it gets alone."""
-    from windtunnel._lib import WT_FLAG_FORCES, WT_FLAG_MASS
-    from windtunnel.batch import Batch
+    from windtunnel._lib import WT_BATCH_PERSISTENT, WT_FLAG_FORCES, WT_FLAG_MASS
     shapes = [(1, 0), (2, 0), (4, 2), (2, 4), (1, 2)]
     alone = _make('fp32', pre_steps=(1,))
     mixed = _make('fp32', pre_steps=(1,))
     for a, m, sh in zip(alone, mixed, shapes):
         a.shape = (1, 0)
         m.shape = sh
-    b = Batch(mixed, PARAMS, own_shapes=True)
+    b = _batch(mixed, PARAMS, persist, own_shapes=True)
     assert b.info.total_tiles == sum(-(-t.n // (256 * t.shape[0])) for t in mixed)
     assert b.info.max_cpt == 4
     for order in (2, 0):
-        b.info.tile_order = order
+        b.info.tile_order = order | (WT_BATCH_PERSISTENT if persist else 0)
         flags = WT_FLAG_FORCES | WT_FLAG_MASS
         for k in range(3):
             b.step(flags)
@@ -178,8 +189,9 @@ def test_batch_mixed_launch_shapes_and_sweep_orders(gpu):
                 assert _same([ta.sums.cpu().numpy()], [bs[j]]), (order, k, j)
 
 
+@pytest.mark.parametrize('persist', [0, 11])
 @pytest.mark.parametrize('form', ['fp16', 'fp32'])
-def test_batch_of_more_than_64_tunnels_equals_alone(gpu, form):
+def test_batch_of_more_than_64_tunnels_equals_alone(gpu, form, persist):
     """70 tunnels of four sizes in one batch: the FP16/BF16 kernel finds a work
     item's tunnel by wave-wide votes over 64 descriptors at a time, so this batch
     needs two of them (FP32 searches); every tunnel ends bit-identical to itself
@@ -203,7 +215,7 @@ def test_batch_of_more_than_64_tunnels_equals_alone(gpu, form):
         alone.append(pair[0])
         mixed.append(pair[1])
         params.append(p)
-    b = Batch(mixed, params)
+    b = _batch(mixed, params, persist)
     assert b.info.ntunnels == 70
     flags = WT_FLAG_FORCES | WT_FLAG_MASS
     for _ in range(3):

@@ -97,7 +97,7 @@ def test_config3_field_error_against_fp64_at_step_200(gpu, c3_sphere):
 
 
 # ---- batched tunnels ----------------------------------------------------------------------
-def _c4_compare(form, oform, shapes=None, steps=3):
+def _c4_compare(form, oform, shapes=None, steps=3, persistent=False):
     from windtunnel import geometry
     from windtunnel._lib import WT_FLAG_FORCES, WT_FLAG_MASS
     from windtunnel.batch import Batch
@@ -108,7 +108,7 @@ def _c4_compare(form, oform, shapes=None, steps=3):
     if shapes:
         for t in tunnels:
             t.shape = shapes[(t.nx, t.ny, t.nz)]
-    b = Batch(tunnels, params, own_shapes=bool(shapes))
+    b = Batch(tunnels, params, own_shapes=bool(shapes), persistent=persistent)
     flags = WT_FLAG_FORCES | WT_FLAG_MASS
     for _ in range(steps):
         b.step(flags)
@@ -126,16 +126,20 @@ def _c4_compare(form, oform, shapes=None, steps=3):
         np.testing.assert_allclose(sums[k, 3], o.mass, rtol=1e-12)
 
 
-def test_config4_batch_bit_exact_with_oracle(gpu):
-    _c4_compare('fp32', O.FP32)
+@pytest.mark.parametrize('persistent', [False, True])
+def test_config4_batch_bit_exact_with_oracle(gpu, persistent):
+    """The 48 tunnels (92.7k work items) by one workgroup per item, and by the
+    persistent kernel, whose workgroups at full occupancy take ~60 items each."""
+    _c4_compare('fp32', O.FP32, persistent=persistent)
 
 
 MIXED = {(64, 32, 32): (1, 0), (96, 48, 48): (2, 0), (128, 64, 64): (4, 2), (160, 80, 80): (2, 6)}
 
 
-@pytest.mark.parametrize('form,oform', [('fp32', O.FP32), ('fp16', O.FP16)])
-def test_config4_mixed_launch_shapes_in_one_launch(gpu, form, oform):
-    _c4_compare(form, oform, shapes=MIXED, steps=2)
+@pytest.mark.parametrize('form,oform,persistent', [('fp32', O.FP32, False), ('fp16', O.FP16, False),
+                                                   ('fp16', O.FP16, True)])
+def test_config4_mixed_launch_shapes_in_one_launch(gpu, form, oform, persistent):
+    _c4_compare(form, oform, shapes=MIXED, steps=2, persistent=persistent)
 
 
 # ---- serving mix ----------------------------------------------------------------------------

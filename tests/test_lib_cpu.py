@@ -127,11 +127,15 @@ def test_flags_and_batch_gates_without_gpu_calls():
     prm = (_lib.Params * nt)(p, p, p)
     cur = (ctypes.c_int32 * nt)(0, 1, 0)
     nbytes, off = lib.wt_batch_workspace_bytes(nt), lib.wt_batch_gate_offset(nt)
-    assert nbytes == off + 8 and off > 0
-    img = (ctypes.c_uint8 * nbytes)()
+    # descriptors, gate bytes padded to 8, then the persistent kernel's 8 claim
+    # counters on 128-byte lines from the next multiple of 128
+    assert off > 0 and nbytes == -(-(off + 8) // 128) * 128 + 8 * 128
+    img = (ctypes.c_uint8 * nbytes)(*([0xa5] * nbytes))
     info = _lib.BatchInfo()
     assert lib.wt_batch_encode(grids, prm, cur, nt, None, img, ctypes.byref(info)) == 0
     assert list(img[off:off + 8]) == [1, 1, 1, 0, 0, 0, 0, 0]
+    assert not any(img[off + 8:nbytes]), 'the claim counters must start at zero'
+    assert info.tile_order == _lib.WT_BATCH_BACKWARDS and info.persist_blocks == 0
     assert info.ntunnels == nt and info.total_tiles == nt * (16 * 8 * 8 // 256)
     # the batch's largest cells-per-thread picks the kernel variant; its tiles follow each shape
     assert info.max_cpt == 1
@@ -147,6 +151,14 @@ def test_flags_and_batch_gates_without_gpu_calls():
     info.max_cpt = 3
     assert lib.wt_batch_step(ctypes.c_void_p(0x9000), ctypes.byref(info), 0, 0, None) < 0
     assert 'cells per thread' in _lib.last_error()
+    info.max_cpt = 1
+    info.tile_order = 8
+    assert lib.wt_batch_step(ctypes.c_void_p(0x9000), ctypes.byref(info), 0, 0, None) < 0
+    assert 'tile order' in _lib.last_error()
+    info.tile_order = _lib.WT_BATCH_PERSISTENT
+    info.persist_blocks = -1
+    assert lib.wt_batch_step(ctypes.c_void_p(0x9000), ctypes.byref(info), 0, 0, None) < 0
+    assert 'persist_blocks' in _lib.last_error()
 
 
 @pytest.mark.parametrize('dims,form,policy', [

@@ -35,7 +35,9 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
 #include <cstdio>
+#include <cstring>
 #include <mutex>
 #include <vector>
 
@@ -48,6 +50,19 @@ namespace wt {
 constexpr int BLOCK = 256;       // threads per workgroup (4 wave64)
 #ifndef WT_STEP_ATTR
 #define WT_STEP_ATTR
+#endif
+// The thread index as the step path reads it. The persistent batched step's unit
+// (wt_step_persist.hip) reads it through an empty asm at every use, so that the
+// compiler cannot hoist what depends on it out of the item loop.
+#ifdef WT_STEP_PERSIST_UNIT
+__device__ __forceinline__ uint32_t opaque_tid() {
+  uint32_t t = threadIdx.x;
+  asm volatile("" : "+v"(t));
+  return t;
+}
+#define WT_TID opaque_tid()
+#else
+#define WT_TID ((uint32_t)threadIdx.x)
 #endif
 constexpr int SLOT = 256;        // cells per reduction slot
 constexpr int CHUNK = 1024;      // slots per first-level reduction block
@@ -217,7 +232,7 @@ __device__ __forceinline__ void stage_link_rows(uint32_t (*rows)[LINK_DW], const
                                                 uint32_t idx0, int32_t nx, int32_t nxy) {
   const __amdgpu_buffer_rsrc_t r =
       __builtin_amdgcn_make_buffer_rsrc((void*)cls, (short)0, (int)n, 0x00020000);
-  const uint32_t lane = threadIdx.x & 63;
+  const uint32_t lane = WT_TID & 63;
   uint32_t* flat = &rows[0][0];            // dword q is row q / LINK_DW, dword q % LINK_DW
   auto offset = [&](uint32_t q) {
     const uint32_t w = q / LINK_DW, k = q - w * LINK_DW;
@@ -245,7 +260,7 @@ __device__ __forceinline__ void stage_link_rows(uint32_t (*rows)[LINK_DW], const
 __device__ __forceinline__ uint32_t staged_code(const uint32_t (*rows)[LINK_DW], uint32_t idx0, int32_t nx,
                                                 int32_t nxy, int dx, int dy, int dz) {
   const int w = (dy + 1) * 3 + (dz + 1);
-  const uint32_t p = (row_start(idx0, w, nx, nxy) & 3u) + (threadIdx.x & 63) + 1u + (uint32_t)dx;
+  const uint32_t p = (row_start(idx0, w, nx, nxy) & 3u) + (WT_TID & 63) + 1u + (uint32_t)dx;
   return reinterpret_cast<const uint8_t*>(rows[w])[p] & CLS_CODE;
 }
 __device__ __forceinline__ Links staged_links(const uint32_t (*rows)[LINK_DW], uint32_t idx0, int32_t nx,
@@ -578,14 +593,22 @@ template <> struct RawIO<uint16_t> {
   }
 };
 
+// A hook the step paths call once per cell after the collision and before the
+// stores, on every lane that steps a cell (at that point a wave has no store
+// outstanding from this cell). The persistent batched step publishes its next
+// claim there; every other kernel passes NoHook.
+struct NoHook {
+  __device__ __forceinline__ void operator()() const {}
+};
+
 // ---- one cell, general path (any mix of solid, face and near-wall lanes) ------------
-template <int FORM, int POL>
+template <int FORM, int POL, class H = NoHook>
 __device__ __forceinline__ void step_cell_general(const StepDesc& d, const typename Form<FORM>::elem* fin,
                                                   typename Form<FORM>::elem* fout,
                                                   const Shell<typename Form<FORM>::real>& shin,
                                                   const Shell<typename Form<FORM>::real>& shout,
                                                   uint32_t idx, uint32_t pc, uint8_t c, int x, int y, int z,
-                                                  bool face, int flags, Acc& acc) {
+                                                  bool face, int flags, Acc& acc, const H& hook = H()) {
   typedef Form<FORM> S;
   typedef typename S::real R;
   typedef typename S::elem E;
@@ -594,8 +617,8 @@ __device__ __forceinline__ void step_cell_general(const StepDesc& d, const typen
   const R zero = (R)0, one = (R)1;
   // class bytes of the wave's neighbour rows, when any lane touches a wall
   __shared__ uint32_t link_rows[BLOCK / 64][9][LINK_DW];
-  uint32_t (*rows)[LINK_DW] = link_rows[threadIdx.x >> 6];
-  const uint32_t idx0 = idx - (threadIdx.x & 63);
+  uint32_t (*rows)[LINK_DW] = link_rows[WT_TID >> 6];
+  const uint32_t idx0 = idx - (WT_TID & 63);
   if (__any((c & CLS_NEAR) != 0)) stage_link_rows(rows, d.cls, d.n, idx0, nx, nxy);
 
   const bool solid = (c & CLS_CODE) != 0;  // solid cell, lbm.py:81-83
@@ -678,6 +701,7 @@ __device__ __forceinline__ void step_cell_general(const StepDesc& d, const typen
     collide_form<FORM>(f, x, nx, (R)d.tau, (R)d.smag, out, ux, uy, uz, rho);
   }
   asm volatile("" : "+v"(sm));
+  hook();
   if (!solid && face) {  // equilibrium boundaries, lbm.py:84-109
     const R m[4] = {sm.x, sm.y, sm.z, sm.w};
     face_state_of<R>(d, m, x, y, z, rho, ux, uy, uz);
@@ -708,13 +732,13 @@ __device__ __forceinline__ void step_cell_general(const StepDesc& d, const typen
 // of population i is the lane's own element offset (one vector register shared
 // by all 38 loads and stores) plus the wave-uniform pshift(i) in a scalar
 // register (raw-buffer offsets, as above), with no per-lane address arithmetic.
-template <int FORM, int POL>
+template <int FORM, int POL, class H = NoHook>
 __device__ __forceinline__ void step_cell_fast(const StepDesc& d, const typename Form<FORM>::elem* fin,
                                                typename Form<FORM>::elem* fout,
                                                const Shell<typename Form<FORM>::real>& shin,
                                                const Shell<typename Form<FORM>::real>& shout, uint32_t idx,
                                                uint32_t pc, int x, int y, int z, bool xface, int flags,
-                                               Acc& acc) {
+                                               Acc& acc, const H& hook = H()) {
   typedef Form<FORM> S;
   typedef typename S::real R;
   typedef typename S::elem E;
@@ -762,6 +786,7 @@ __device__ __forceinline__ void step_cell_fast(const StepDesc& d, const typename
   // the boundary arithmetic may not move above this point (the compiler would
   // otherwise hoist it next to the snapshot loads, and wait for them there)
   asm volatile("" : "+v"(sm));
+  hook();
   if (xface) {
     // inlet / outlet lanes of this wave (lbm.py:84-109)
     const R m[4] = {sm.x, sm.y, sm.z, sm.w};
@@ -794,12 +819,12 @@ __device__ __forceinline__ void step_cell_fast(const StepDesc& d, const typename
   if (flags & WT_FLAG_MASS) acc.m += (double)rho;
 }
 
-template <int FORM, int POL>
+template <int FORM, int POL, class H = NoHook>
 __device__ __forceinline__ void step_cell(const StepDesc& d, const typename Form<FORM>::elem* fin,
                                           typename Form<FORM>::elem* fout,
                                           const Shell<typename Form<FORM>::real>& shin,
                                           const Shell<typename Form<FORM>::real>& shout,
-                                          uint32_t idx, int flags, Acc& acc) {
+                                          uint32_t idx, int flags, Acc& acc, const H& hook = H()) {
   const int32_t nx = d.nx, ny = d.ny, nz = d.nz;
   const uint32_t yz = divu(idx, d.mx, d.sx);
   const int x = (int)(idx - yz * (uint32_t)nx);
@@ -813,9 +838,9 @@ __device__ __forceinline__ void step_cell(const StepDesc& d, const typename Form
   // fluid lanes away from walls, and fluid inlet/outlet lanes, take the fast path
   const bool simple = (c == 0) & (!face | xface) & (y != 0) & (y != ny - 1) & (z != 0) & (z != nz - 1);
   if (__all(simple))
-    step_cell_fast<FORM, POL>(d, fin, fout, shin, shout, idx, pc, x, y, z, xface, flags, acc);
+    step_cell_fast<FORM, POL, H>(d, fin, fout, shin, shout, idx, pc, x, y, z, xface, flags, acc, hook);
   else
-    step_cell_general<FORM, POL>(d, fin, fout, shin, shout, idx, pc, c, x, y, z, face, flags, acc);
+    step_cell_general<FORM, POL, H>(d, fin, fout, shin, shout, idx, pc, c, x, y, z, face, flags, acc, hook);
 }
 
 // ---- deterministic FP64 reduction ------------------------------------------------
@@ -830,7 +855,7 @@ __device__ __forceinline__ double wave_sum(double v) {
 // sums in wave order.
 __device__ __forceinline__ void slot_stage(double (*lds)[4][4], int k, Acc a) {
   const double fx = wave_sum(a.fx), fy = wave_sum(a.fy), fz = wave_sum(a.fz), m = wave_sum(a.m);
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int lane = WT_TID & 63, wave = WT_TID >> 6;
   if (lane == 0) {
     lds[k][wave][0] = fx; lds[k][wave][1] = fy; lds[k][wave][2] = fz; lds[k][wave][3] = m;
   }
@@ -840,7 +865,7 @@ template <int CPT>
 __device__ __forceinline__ void slot_flush(double (*lds)[4][4], double* partials, uint32_t slot0,
                                            uint32_t nslots) {
   __syncthreads();
-  const int t = threadIdx.x;
+  const int t = WT_TID;
   if (t < 4 * CPT) {
     const int k = t >> 2, comp = t & 3;
     if (slot0 + k < nslots) {
@@ -859,12 +884,17 @@ __device__ __forceinline__ void slot_flush(double (*lds)[4][4], double* partials
 //      tunnels lose with it);
 //   2  one tunnel whose two population buffers fit in 224 MiB: everything cached,
 //      so the Infinity Cache carries a step's output into the next step.
-template <int FORM, int CPT, int POL>
-__device__ __forceinline__ void run_tile(const StepDesc& d, int parity, uint32_t tile, int flags) {
+// NBUF slot staging areas in LDS, area lbuf for this tile: the persistent
+// batched step alternates two, so that the next tile's staging needs no barrier
+// after this tile's flush.
+template <int FORM, int CPT, int POL, int NBUF = 1, class H = NoHook>
+__device__ __forceinline__ void run_tile(const StepDesc& d, int parity, uint32_t tile, int flags, int lbuf = 0,
+                                         const H& hook = H()) {
   typedef Form<FORM> S;
   typedef typename S::real R;
   typedef typename S::elem E;
-  __shared__ double lds[CPT][4][4];
+  __shared__ double slots[NBUF][CPT][4][4];
+  double (*lds)[4][4] = slots[NBUF > 1 ? lbuf : 0];
   const int side = parity & 1;
   const E* fin = (const E*)(side ? d.fout : d.fin);
   E* fout = (E*)(side ? d.fin : d.fout);
@@ -873,9 +903,9 @@ __device__ __forceinline__ void run_tile(const StepDesc& d, int parity, uint32_t
   const uint32_t base = tile * (uint32_t)(BLOCK * CPT);
 #pragma unroll
   for (int k = 0; k < CPT; k++) {
-    const uint32_t idx = base + (uint32_t)(k * BLOCK) + threadIdx.x;
+    const uint32_t idx = base + (uint32_t)(k * BLOCK) + WT_TID;
     Acc a{0.0, 0.0, 0.0, 0.0};
-    if (idx < d.n) step_cell<FORM, POL>(d, fin, fout, shin, shout, idx, flags, a);
+    if (idx < d.n) step_cell<FORM, POL, H>(d, fin, fout, shin, shout, idx, flags, a, hook);
     if (flags & (WT_FLAG_FORCES | WT_FLAG_MASS)) slot_stage(lds, k, a);
   }
   if (flags & (WT_FLAG_FORCES | WT_FLAG_MASS))
@@ -938,11 +968,26 @@ __global__ __launch_bounds__(BLOCK) WT_STEP_ATTR void k_step_adv(StepDesc d, Adv
 // one-cell variant (64 registers, no spills; measured 1.6 % faster than its
 // natural 7), no constraint otherwise.
 template <int FORM, int MAXCPT> constexpr int batch_min_waves() { return Form<FORM>::dev && MAXCPT == 1 ? 8 : 1; }
-template <int FORM, int MAXCPT>
-__global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(batch_min_waves<FORM, MAXCPT>())))
-void k_step_batch(const StepDesc* __restrict__ ds, const uint8_t* __restrict__ gates, int nd, uint32_t nitems,
-                  int parity, int backwards, int flags) {
-  const uint32_t item = backwards ? nitems - 1u - blockIdx.x : blockIdx.x;
+
+// gfx950's constant address space (scalar loads of memory no launch writes); the
+// host pass of this single-source file sees plain pointers
+#ifdef __HIP_DEVICE_COMPILE__
+#define WT_CONSTANT_AS __attribute__((address_space(4)))
+#else
+#define WT_CONSTANT_AS
+#endif
+
+// One work item of a batched step: finds the item's tunnel and runs its tile.
+template <int FORM, int MAXCPT, int NBUF = 1, class H = NoHook>
+__device__ __forceinline__ void batch_item(const StepDesc* __restrict__ ds, const uint8_t* __restrict__ gates,
+                                          int nd, uint32_t item, int parity, int flags, int lbuf = 0,
+                                          const H& hook = H()) {
+  // the descriptor table and gate bytes are written before the launch and only read
+  // by it: reading them through the constant address space lets the compiler keep
+  // the item's descriptor in scalar registers inside the persistent kernel's loop too
+  typedef const WT_CONSTANT_AS StepDesc* CDesc;
+  typedef const WT_CONSTANT_AS uint32_t* CWord;
+  const CDesc cds = (CDesc)ds;
   // the item's tunnel: for FP16/BF16 the number of tunnels whose first tile is at
   // or before it, less one (ds[0].tile_begin is 0), counted 64 tunnels per
   // wave-wide vote: one round of loads instead of a chain of dependent ones
@@ -952,34 +997,162 @@ void k_step_batch(const StepDesc* __restrict__ ds, const uint8_t* __restrict__ g
   if constexpr (Form<FORM>::dev) {
     lo = -1;
     for (int c0 = 0; c0 < nd; c0 += 64) {
-      const int k = c0 + (int)(threadIdx.x & 63);
-      lo += __popcll(__ballot(k < nd && ds[k].tile_begin <= item));
+      const int k = c0 + (int)(WT_TID & 63);
+      lo += __popcll(__ballot(k < nd && cds[k].tile_begin <= item));
     }
   } else {
     int hi = nd - 1;
     while (lo < hi) {
       const int mid = (lo + hi + 1) >> 1;
-      if (ds[mid].tile_begin <= item) lo = mid; else hi = mid - 1;
+      if (cds[mid].tile_begin <= item) lo = mid; else hi = mid - 1;
     }
   }
-  const StepDesc d = ds[lo];
+  const StepDesc d = cds[lo];
   const int p = parity ^ (d.flags & 1);
   const uint32_t tile = item - d.tile_begin;
   // with WT_FLAG_NO_MAC a tunnel whose gate is set still stores its macroscopic field
   // (read as the gate's aligned word, a scalar load: the gate bytes are padded to a
   // multiple of 8 and start 8-byte aligned in the workspace)
-  const uint32_t gw = reinterpret_cast<const uint32_t*>(gates)[lo >> 2];
+  const uint32_t gw = ((CWord)gates)[lo >> 2];
   const int fl = ((flags & WT_FLAG_NO_MAC) && ((gw >> (8 * (lo & 3))) & 0xffu)) ? (flags & ~WT_FLAG_NO_MAC) : flags;
   if constexpr (MAXCPT >= 4) {
-    if (d.cpt == 4) { run_tile<FORM, 4, 0>(d, p, tile, fl); return; }
+    if (d.cpt == 4) { run_tile<FORM, 4, 0, NBUF, H>(d, p, tile, fl, lbuf, hook); return; }
   }
   if constexpr (MAXCPT >= 2) {
-    if (d.cpt == 2) { run_tile<FORM, 2, 0>(d, p, tile, fl); return; }
+    if (d.cpt == 2) { run_tile<FORM, 2, 0, NBUF, H>(d, p, tile, fl, lbuf, hook); return; }
   }
-  run_tile<FORM, 1, 0>(d, p, tile, fl);
+  run_tile<FORM, 1, 0, NBUF, H>(d, p, tile, fl, lbuf, hook);
 }
 
-#ifndef WT_STEP_FP32_UNIT  // kernels launched only from the main unit
+template <int FORM, int MAXCPT>
+__global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(batch_min_waves<FORM, MAXCPT>())))
+void k_step_batch(const StepDesc* __restrict__ ds, const uint8_t* __restrict__ gates, int nd, uint32_t nitems,
+                  int parity, int backwards, int flags) {
+  const uint32_t item = backwards ? nitems - 1u - blockIdx.x : blockIdx.x;
+  batch_item<FORM, MAXCPT>(ds, gates, nd, item, parity, flags);
+}
+
+// Persistent batched step (WT_BATCH_PERSISTENT): a grid of the device's full
+// occupancy whose workgroups claim the items dynamically, in order, from nctr
+// interleaved counters (nctr a power of 2, at most 8 and at most the
+// workgroups; workgroup b takes items k, k + nctr, k + 2 nctr, ... of counter
+// k = b % nctr; workgroups are dispatched to the 8 XCDs round-robin, so with 8
+// counters each XCD's workgroups share one). The items in flight then form one
+// compact front through the batch, as with one workgroup per item (a static
+// stride over the items, where workgroups drift apart, measured 19 % slower).
+// A workgroup's first item is its own; it claims the next while it runs the
+// current one. Every workgroup's last claim(s) fail (exceed its counter's
+// items); the one that draws the last value of all resets the counter to 0 for
+// the next launch, and the host encodes the counters as 0 in the first place, so
+// the kernel can be captured and replayed. The arguments are re-read from the
+// argument segment on every item, so that nothing but the claims is live across
+// the item loop (with the unit's build options: the per-item code's registers).
+constexpr int BATCH_COUNTERS = 8;
+constexpr int BATCH_COUNTER_STRIDE = 32;  // words: one 128-byte line per counter
+struct BatchArgs {
+  const StepDesc* ds;
+  const uint8_t* gates;
+  uint32_t* counters;
+  int32_t nd;
+  uint32_t nitems;
+  int32_t parity;
+  int32_t backwards;
+  int32_t flags;
+  uint32_t nblocks;   // workgroups of the launch
+  uint32_t log_nctr;  // counters in use: 2^log_nctr <= min(BATCH_COUNTERS, nblocks)
+};
+
+// The claims of an FP32 workgroup come from a fifth wave that runs no tile (CW):
+// the compiler makes any use of an atomic's result wait for every store the wave
+// has outstanding, so a tile wave taking the claims stalls on its own population
+// stores. The other forms, which run more waves per SIMD, let thread 0 of the
+// first tile wave claim.
+// Measured on the 48-tunnel batch: FP32 with the claim wave 2 % slower than one
+// workgroup per item, with thread 0 claiming 6 % slower; FP16 with thread 0 claiming
+// 8-10 % slower, with the claim wave 22 % slower (its fifth wave takes a quarter of
+// the tile waves' occupancy: 8 waves per SIMD at 64 registers).
+constexpr int CLAIM_WAVE_THREADS = 64;
+template <bool CW> constexpr int persist_block() { return CW ? BLOCK + CLAIM_WAVE_THREADS : BLOCK; }
+template <int FORM> constexpr bool persist_claim_wave() { return FORM == WT_FP32; }
+
+// Thread 0's claims, taken in the tile code's pre-store hook (the item's loads
+// are in and its stores not yet out, so the claim's atomic neither holds the
+// loads back nor waits for the stores): the hook publishes the claim that came
+// back since the previous item's hook and sends the next one, once per item.
+struct ClaimHook {
+  uint32_t* slot;
+  uint32_t* pend;
+  bool* fired;
+  uint32_t* ctr;
+  uint32_t base;
+  bool active;
+  __device__ __forceinline__ void operator()() const {
+    if (active && !*fired) {
+      *slot = *pend;
+      *pend = base + atomicAdd(ctr, 1u);
+      *fired = true;
+    }
+  }
+};
+
+template <int FORM, int MAXCPT, bool CW>
+__global__ __launch_bounds__(persist_block<CW>()) __attribute__((amdgpu_waves_per_eu(batch_min_waves<FORM, MAXCPT>())))
+void k_step_batch_persist(BatchArgs args) {
+  __shared__ uint32_t claim[2];
+  // tile waves 0-3, the claim wave 4 (a wave-uniform value: each wave takes one side)
+  const bool claimer = CW && __builtin_amdgcn_readfirstlane(WT_TID >> 6) == BLOCK / 64;
+  const uint32_t claim_tid = CW ? BLOCK : 0;
+  // a workgroup's first item is its own (j = b / nctr of counter b % nctr, one per
+  // workgroup: there are at least as many items as workgroups); the claims then
+  // hand out the rest of each counter's items in order (claim value v: j = wk + v).
+  // Values are read into scalar registers: the item, its tunnel and its
+  // descriptor are workgroup-uniform, and the tile code addresses through them.
+  // Without the claim wave thread 0 keeps one claim in flight beyond the one it
+  // publishes (sent at the start, then one per item from the hook), so each
+  // workgroup's last two claims fail; with it, the claim wave sends one per item
+  // and each workgroup's last claim fails.
+  uint32_t j, pend = 0;
+  {
+    const WT_CONSTANT_AS BatchArgs* a0 = (const WT_CONSTANT_AS BatchArgs*)__builtin_amdgcn_kernarg_segment_ptr();
+    const uint32_t ln = a0->log_nctr, nc = 1u << ln, k = blockIdx.x & (nc - 1u);
+    j = blockIdx.x >> ln;
+    if (!CW && WT_TID == 0)
+      pend = ((a0->nblocks - k + nc - 1u) >> ln) + atomicAdd(a0->counters + BATCH_COUNTER_STRIDE * k, 1u);
+  }
+  for (int it = 0;; it ^= 1) {
+    const WT_CONSTANT_AS BatchArgs* a = (const WT_CONSTANT_AS BatchArgs*)__builtin_amdgcn_kernarg_segment_ptr();
+    asm volatile("" : "+s"(a));
+    const uint32_t ln = a->log_nctr, nc = 1u << ln, k = blockIdx.x & (nc - 1u), nitems = a->nitems;
+    uint32_t* ctr = a->counters + BATCH_COUNTER_STRIDE * k;
+    const uint32_t nk = (nitems - k + nc - 1u) >> ln;    // items k, k + nc, ... below nitems
+    const uint32_t wk = (a->nblocks - k + nc - 1u) >> ln;  // workgroups sharing the counter
+    if (j >= nk) {
+      // the last claim of all (value nk - 1, resp. nk + wk - 1) resets the counter
+      // for the next launch
+      if (CW ? (WT_TID == claim_tid && j == nk + wk - 1u) : (WT_TID == 0 && pend == nk + 2u * wk - 1u))
+        atomicExch(ctr, 0u);
+      break;
+    }
+    const int flags = a->flags;
+    // the next claim reaches LDS before the item's barrier (the tile's slot flush
+    // when it reduces forces or mass, otherwise an explicit one); claim slots
+    // alternate, so a slot is rewritten only after everyone has read it
+    if (claimer) {
+      if (WT_TID == claim_tid) claim[it] = wk + atomicAdd(ctr, 1u);
+      __syncthreads();
+    } else {
+      bool fired = false;
+      const ClaimHook hook{&claim[it], &pend, &fired, ctr, wk, !CW && WT_TID == 0};
+      const uint32_t item = k + (j << ln);
+      batch_item<FORM, MAXCPT, 2, ClaimHook>(a->ds, a->gates, a->nd, a->backwards ? nitems - 1u - item : item,
+                                             a->parity, flags, it, hook);
+      if (!(flags & (WT_FLAG_FORCES | WT_FLAG_MASS))) __syncthreads();
+    }
+    j = __builtin_amdgcn_readfirstlane(claim[it]);
+  }
+}
+
+#if !defined(WT_STEP_FP32_UNIT) && !defined(WT_STEP_PERSIST_UNIT)  // kernels launched only from the main unit
 // ---- reductions of the slots ---------------------------------------------------------
 struct RedDesc {
   double* partials;  // nslots slots, then nchunks first-level sums
@@ -1256,9 +1429,72 @@ void launch_step_adv_f(const StepDesc& d, const AdvectArgs& a, Shape sh, int fla
 // whatever its register count. FP16/BF16 take their 7 waves (65 registers).
 constexpr size_t FP32_BATCH_PAD_LDS = 22 * 1024;
 template <int FORM> constexpr size_t batch_pad_lds() { return FORM == WT_FP32 ? FP32_BATCH_PAD_LDS : 0; }
+// Workgroups of a persistent launch: as many as the device holds at once (the
+// kernel's occupancy per CU times the CUs), found once per device and kernel.
+template <typename K>
+static uint32_t resident_blocks(K kernel, int block, size_t lds, std::atomic<int>* cache, uint32_t nitems) {
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) return nitems;
+  int g = cache[dev].load(std::memory_order_relaxed);
+  if (g <= 0) {
+    int per_cu = 0, cus = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, block, lds) != hipSuccess ||
+        hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || per_cu <= 0 ||
+        cus <= 0)
+      return nitems;
+    g = per_cu * cus;
+    cache[dev].store(g, std::memory_order_relaxed);
+  }
+  return (uint32_t)g < nitems ? (uint32_t)g : nitems;
+}
+
+// Persistent batched step: a grid of one full occupancy, each workgroup striding
+// over the items (instantiated in wt_step_persist.hip).
+template <int FORM, int MAXCPT, bool CW>
+static void launch_batch_persist_t(const StepDesc* dd, const uint8_t* gates, uint32_t* counters, int nd,
+                                   uint32_t nitems, int parity, int backwards, int flags, uint32_t cap,
+                                   hipStream_t s) {
+  static std::atomic<int> cache[64];
+  const size_t pad = batch_pad_lds<FORM>();
+  uint32_t g = resident_blocks(k_step_batch_persist<FORM, MAXCPT, CW>, persist_block<CW>(), pad, cache, nitems);
+  if (cap > 0 && cap < g) g = cap;
+  uint32_t log_nctr = 0;
+  while ((2u << log_nctr) <= g && (2u << log_nctr) <= (uint32_t)BATCH_COUNTERS) log_nctr++;
+  const BatchArgs a{dd, gates, counters, nd, nitems, parity, backwards, flags, g, log_nctr};
+  hipLaunchKernelGGL((k_step_batch_persist<FORM, MAXCPT, CW>), dim3(g), dim3(persist_block<CW>()), pad, s, a);
+}
 template <int FORM>
-void launch_batch_f(const StepDesc* dd, const uint8_t* gates, int nd, uint32_t nitems, int parity, int backwards,
-                    int flags, int max_cpt, hipStream_t s) {
+void launch_batch_persist_f(const StepDesc* dd, const uint8_t* gates, uint32_t* counters, int nd, uint32_t nitems,
+                            int parity, int backwards, int flags, int max_cpt, uint32_t cap, hipStream_t s) {
+  constexpr bool CW = persist_claim_wave<FORM>();
+  if (max_cpt >= 4)
+    launch_batch_persist_t<FORM, 4, CW>(dd, gates, counters, nd, nitems, parity, backwards, flags, cap, s);
+  else if (max_cpt == 2)
+    launch_batch_persist_t<FORM, 2, CW>(dd, gates, counters, nd, nitems, parity, backwards, flags, cap, s);
+  else
+    launch_batch_persist_t<FORM, 1, CW>(dd, gates, counters, nd, nitems, parity, backwards, flags, cap, s);
+}
+
+#define WT_BATCH_PERSIST_DECL(F)                                                                            \
+  void launch_batch_persist_f<F>(const StepDesc* dd, const uint8_t* gates, uint32_t* counters, int nd,          \
+                                 uint32_t nitems, int parity, int backwards, int flags, int max_cpt, uint32_t cap, \
+                                 hipStream_t s)
+#ifndef WT_STEP_PERSIST_UNIT
+extern template WT_BATCH_PERSIST_DECL(WT_FP32);
+extern template WT_BATCH_PERSIST_DECL(WT_FP16);
+extern template WT_BATCH_PERSIST_DECL(WT_BF16);
+extern template WT_BATCH_PERSIST_DECL(WT_FP64);
+#endif
+
+// One workgroup per item, or the persistent form (with at most cap workgroups
+// when cap is positive).
+template <int FORM>
+void launch_batch_f(const StepDesc* dd, const uint8_t* gates, uint32_t* counters, int nd, uint32_t nitems,
+                    int parity, int backwards, int flags, int max_cpt, int persist, uint32_t cap, hipStream_t s) {
+  if (persist) {
+    launch_batch_persist_f<FORM>(dd, gates, counters, nd, nitems, parity, backwards, flags, max_cpt, cap, s);
+    return;
+  }
   const size_t pad = batch_pad_lds<FORM>();
   if (max_cpt >= 4)
     hipLaunchKernelGGL((k_step_batch<FORM, 4>), dim3(nitems), dim3(BLOCK), pad, s, dd, gates, nd, nitems, parity,
@@ -1282,6 +1518,13 @@ void launch_batch_f(const StepDesc* dd, const uint8_t* gates, int nd, uint32_t n
 template void launch_step_f<WT_FP32>(const StepDesc& d, int parity, Shape sh, int flags, hipStream_t s);
 template void launch_step_adv_f<WT_FP32>(const StepDesc& d, const AdvectArgs& a, Shape sh, int flags,
                                         hipStream_t s);
+
+}  // namespace wt
+#elif defined(WT_STEP_PERSIST_UNIT)
+template WT_BATCH_PERSIST_DECL(WT_FP32);
+template WT_BATCH_PERSIST_DECL(WT_FP16);
+template WT_BATCH_PERSIST_DECL(WT_BF16);
+template WT_BATCH_PERSIST_DECL(WT_FP64);
 
 }  // namespace wt
 #else
@@ -1603,13 +1846,20 @@ int wt_link_classes(const wt_grid* g, uint64_t* out_dev, void* stream) {
 }
 
 // ---- batched tunnels ---------------------------------------------------------------------
-// Workspace layout: [StepDesc x ntunnels][RedDesc x ntunnels].
+// Workspace layout: [StepDesc x ntunnels][RedDesc x ntunnels][gate bytes, padded
+// to 8][up to 128 zero bytes][the persistent kernel's claim counters, one per
+// 128-byte line].
+static int64_t batch_counter_offset(int ntunnels) {
+  const int64_t end = wt_batch_gate_offset(ntunnels) + (((int64_t)ntunnels + 7) & ~(int64_t)7);
+  return (end + 127) & ~(int64_t)127;
+}
+
 int64_t wt_batch_workspace_bytes(int ntunnels) {
   if (ntunnels <= 0) {
     fail(-1, "ntunnels must be positive");
     return -1;
   }
-  return wt_batch_gate_offset(ntunnels) + (((int64_t)ntunnels + 7) & ~(int64_t)7);
+  return batch_counter_offset(ntunnels) + (int64_t)BATCH_COUNTERS * BATCH_COUNTER_STRIDE * 4;
 }
 
 int64_t wt_batch_gate_offset(int ntunnels) {
@@ -1658,13 +1908,16 @@ int wt_batch_encode(const wt_grid* grids, const wt_params* params, const int* cu
     gates[k] = 1;  // every tunnel stores its macroscopic field until the caller says otherwise
   }
   for (int64_t k = ntunnels; k < ((ntunnels + 7) & ~7); k++) gates[k] = 0;
+  // the persistent kernel's claim counters start at 0 (and it leaves them at 0)
+  std::memset((char*)host_image + wt_batch_gate_offset(ntunnels) + ((ntunnels + 7) & ~7), 0,
+              (size_t)(wt_batch_workspace_bytes(ntunnels) - wt_batch_gate_offset(ntunnels) - ((ntunnels + 7) & ~7)));
   info->total_tiles = (int64_t)tile;
   info->max_chunks = (int32_t)max_chunks;
   info->form = form;
   info->ntunnels = ntunnels;
-  info->tile_order = 2;
+  info->tile_order = WT_BATCH_BACKWARDS;
   info->max_cpt = max_cpt;
-  info->reserved = 0;
+  info->persist_blocks = 0;
   return 0;
 }
 
@@ -1690,7 +1943,8 @@ int wt_batch_step(void* workspace, const wt_batch_info* info, int parity, int fl
   if ((uintptr_t)workspace & 15) return fail(-1, "batch workspace is not 16-byte aligned");
   const int form = info->form;
   if (form < WT_FP32 || form > WT_FP64) return fail(-1, "unknown storage form %d", form);
-  if (info->tile_order & ~2) return fail(-1, "unknown batch tile order bits 0x%x", info->tile_order);
+  if (info->tile_order & ~(WT_BATCH_BACKWARDS | WT_BATCH_PERSISTENT))
+    return fail(-1, "unknown batch tile order bits 0x%x", info->tile_order);
   if (info->max_cpt != 1 && info->max_cpt != 2 && info->max_cpt != 4)
     return fail(-1, "batch info has cells per thread %d (not set by wt_batch_encode?)", info->max_cpt);
   if (flags & ~(WT_FLAG_FORCES | WT_FLAG_MASS | WT_FLAG_NO_MAC)) return fail(-1, "unknown flag bits 0x%x", flags);
@@ -1699,14 +1953,18 @@ int wt_batch_step(void* workspace, const wt_batch_info* info, int parity, int fl
   const StepDesc* dd = (const StepDesc*)workspace;
   const RedDesc* rd = (const RedDesc*)((const char*)workspace + sizeof(StepDesc) * nt);
   const uint8_t* gates = (const uint8_t*)workspace + wt_batch_gate_offset(nt);
+  uint32_t* ctrs = (uint32_t*)((char*)workspace + batch_counter_offset(nt));
   const int f = flags & (WT_FLAG_FORCES | WT_FLAG_MASS | WT_FLAG_NO_MAC);
   const uint32_t nitems = (uint32_t)info->total_tiles;
-  const int backwards = ((info->tile_order & 2) && (parity & 1)) ? 1 : 0;
+  const int backwards = ((info->tile_order & WT_BATCH_BACKWARDS) && (parity & 1)) ? 1 : 0;
+  if (info->persist_blocks < 0) return fail(-1, "negative persist_blocks %d", info->persist_blocks);
+  const int persist = (info->tile_order & WT_BATCH_PERSISTENT) ? 1 : 0;
+  const uint32_t cap = (uint32_t)info->persist_blocks;
   switch (form) {
-    case WT_FP32: launch_batch_f<WT_FP32>(dd, gates, nt, nitems, parity, backwards, f, info->max_cpt, s); break;
-    case WT_FP16: launch_batch_f<WT_FP16>(dd, gates, nt, nitems, parity, backwards, f, info->max_cpt, s); break;
-    case WT_BF16: launch_batch_f<WT_BF16>(dd, gates, nt, nitems, parity, backwards, f, info->max_cpt, s); break;
-    case WT_FP64: launch_batch_f<WT_FP64>(dd, gates, nt, nitems, parity, backwards, f, info->max_cpt, s); break;
+    case WT_FP32: launch_batch_f<WT_FP32>(dd, gates, ctrs, nt, nitems, parity, backwards, f, info->max_cpt, persist, cap, s); break;
+    case WT_FP16: launch_batch_f<WT_FP16>(dd, gates, ctrs, nt, nitems, parity, backwards, f, info->max_cpt, persist, cap, s); break;
+    case WT_BF16: launch_batch_f<WT_BF16>(dd, gates, ctrs, nt, nitems, parity, backwards, f, info->max_cpt, persist, cap, s); break;
+    case WT_FP64: launch_batch_f<WT_FP64>(dd, gates, ctrs, nt, nitems, parity, backwards, f, info->max_cpt, persist, cap, s); break;
   }
   if (f & (WT_FLAG_FORCES | WT_FLAG_MASS)) {
     hipLaunchKernelGGL(k_reduce1, dim3((unsigned)info->max_chunks, (unsigned)nt), dim3(BLOCK), 0, s, rd);

@@ -19,6 +19,8 @@ WT_FP32, WT_FP16, WT_BF16, WT_FP64 = 0, 1, 2, 3
 FORMS = {'fp32': WT_FP32, 'fp16': WT_FP16, 'bf16': WT_BF16, 'fp64': WT_FP64}
 FORM_NAMES = {v: k for k, v in FORMS.items()}
 WT_FLAG_FORCES, WT_FLAG_MASS, WT_FLAG_NO_MAC = 1, 2, 4
+# wt_batch_info.tile_order bits
+WT_BATCH_BACKWARDS, WT_BATCH_PERSISTENT = 2, 4
 
 c_i32, c_i64, c_u32, c_f32, c_vp = ctypes.c_int32, ctypes.c_int64, ctypes.c_uint32, ctypes.c_float, ctypes.c_void_p
 
@@ -43,7 +45,7 @@ class SeedBox(ctypes.Structure):
 class BatchInfo(ctypes.Structure):
     """wt_batch_info"""
     _fields_ = [('total_tiles', c_i64), ('max_chunks', c_i32), ('form', c_i32), ('ntunnels', c_i32),
-                ('tile_order', c_i32), ('max_cpt', c_i32), ('reserved', c_i32)]
+                ('tile_order', c_i32), ('max_cpt', c_i32), ('persist_blocks', c_i32)]
 
 
 P = ctypes.POINTER

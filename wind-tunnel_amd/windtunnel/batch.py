@@ -8,6 +8,10 @@ the device in stream order, with no synchronization; each step is then a
 kernel launch only (wt_batch_step), so a step of the whole batch can be
 captured in a HIP graph. Graph replays advance two steps (parity 0 then 1),
 returning every tunnel to the same buffer parity.
+
+A step runs either one workgroup per (tunnel, tile) item, or, with
+persistent=True, a grid of one full occupancy whose workgroups stride over the
+items (the persistent kernel). Both compute every cell identically.
 """
 import ctypes
 
@@ -16,14 +20,19 @@ import torch
 from . import _lib, autotune
 from ._lib import BatchInfo, Grid, Params, check, stream_handle
 
+# the kernel a Batch steps with unless told otherwise
+PERSISTENT = False
+
 
 class Batch:
-    def __init__(self, tunnels, params, cpt=None, own_shapes=False):
+    def __init__(self, tunnels, params, cpt=None, own_shapes=False, persistent=None, persist_blocks=0):
         """tunnels: list of Tunnel (same storage form); params: list of (uin, tau, smag).
 
         Every tunnel runs at cpt cells per thread (default: the value tuned for
         batches of this storage form, autotune.batch_cpt), unless own_shapes, when
-        each keeps the cells per thread of its own launch shape."""
+        each keeps the cells per thread of its own launch shape. persistent picks
+        the persistent kernel (default: PERSISTENT), run with persist_blocks
+        workgroups if positive, else at the device's full occupancy."""
         if not tunnels:
             raise ValueError('empty batch')
         forms = {t.form for t in tunnels}
@@ -34,6 +43,10 @@ class Batch:
         self.cpt = None if own_shapes else int(cpt if cpt is not None else
                                                 autotune.batch_cpt(_lib.FORM_NAMES[self.tunnels[0].form]))
         self.params = [tuple(float(v) for v in p) for p in params]
+        self.persistent = PERSISTENT if persistent is None else bool(persistent)
+        self.persist_blocks = int(persist_blocks)
+        if self.persist_blocks < 0:
+            raise ValueError(f'persist_blocks must be >= 0, got {persist_blocks}')
         nt = len(self.tunnels)
         nbytes = check(self.lib.wt_batch_workspace_bytes(nt), 'wt_batch_workspace_bytes')
         self.workspace = torch.empty(nbytes, dtype=torch.uint8, device=self.tunnels[0].device)
@@ -64,6 +77,9 @@ class Batch:
         check(self.lib.wt_batch_encode(grids, prm, cur, nt, ctypes.c_void_p(self.sums.data_ptr()),
                                        ctypes.c_void_p(image.data_ptr()), ctypes.byref(self.info)),
               'wt_batch_encode')
+        if self.persistent:
+            self.info.tile_order |= _lib.WT_BATCH_PERSISTENT
+            self.info.persist_blocks = self.persist_blocks
         s = stream if stream is not None else torch.cuda.current_stream(self.workspace.device)
         with torch.cuda.stream(s):
             self.workspace.copy_(image, non_blocking=True)

@@ -19,10 +19,15 @@ INCLUDE = os.path.join(REPO, 'include')
 LIB = os.path.join(PKG, 'libwt.so')
 ARCH = os.environ.get('WT_OFFLOAD_ARCH', 'gfx950')
 
-HIP_SOURCES = ['wt_step.hip', 'wt_step_fp32.hip', 'wt_aux.hip']
+HIP_SOURCES = ['wt_step.hip', 'wt_step_fp32.hip', 'wt_step_persist.hip', 'wt_aux.hip']
 # per-source compiler options: the FP32 step kernels schedule for instruction-level
 # parallelism (measured 1.7 % faster at 33.5M cells; the FP16 kernels 8 % slower)
-SOURCE_FLAGS = {'wt_step_fp32.hip': ['-mllvm', '-amdgpu-sched-strategy=max-ilp']}
+SOURCE_FLAGS = {'wt_step_fp32.hip': ['-mllvm', '-amdgpu-sched-strategy=max-ilp'],
+                # the persistent batched step: no machine loop-invariant code motion, which
+                # would keep the per-item code's constants live across the item loop, and
+                # no wave-level atomic rewriting, which would wait for each claim at once
+                'wt_step_persist.hip': ['-mllvm', '-disable-machine-licm',
+                                        '-mllvm', '-amdgpu-atomic-optimizer-strategy=None']}
 CPP_SOURCES = ['wt_voxel.cpp']
 
 COMMON = ['-O3', '-std=c++17', '-fPIC', '-ffp-contract=off', '-Wall', '-Wno-unused-function',

@@ -50,12 +50,12 @@ def build_config4(form='fp32', seed=4, ntunnels=48):
     return tunnels, params
 
 
-def run_config4(steps=1000, warmup=20, form='fp32', graphs=False, seed=4, store_mac=False):
+def run_config4(steps=1000, warmup=20, form='fp32', graphs=False, seed=4, store_mac=False, persistent=None):
     """The workload asks for forces every step and nothing that reads the
     macroscopic field, so by default no step stores it (WT_FLAG_NO_MAC, every gate
     closed); store_mac=True stores it every step as the reference's step does."""
     tunnels, params = build_config4(form, seed)
-    b = Batch(tunnels, params)
+    b = Batch(tunnels, params, persistent=persistent)
     flags = WT_FLAG_FORCES
     if not store_mac:
         flags |= WT_FLAG_NO_MAC
@@ -87,6 +87,7 @@ def run_config4(steps=1000, warmup=20, form='fp32', graphs=False, seed=4, store_
     torch.cuda.synchronize()
     wall = time.perf_counter() - h0
     return {'tunnels': len(tunnels), 'cells': cells, 'steps': done, 'graphs': graphs, 'store_mac': store_mac,
+            'persistent': b.persistent,
             'wall_s': wall, 'gpu_ms': e0.elapsed_time(e1), 'mlups': cells * done / wall / 1e6,
             'ms_per_step': wall * 1e3 / done}
 
@@ -100,9 +101,11 @@ def main(argv=None):
     ap.add_argument('--form', default='fp32')
     ap.add_argument('--no-graphs', action='store_true')
     ap.add_argument('--store-mac', action='store_true', help='store the macroscopic field every step')
+    ap.add_argument('--persistent', type=int, choices=(0, 1), default=None,
+                    help='1: the persistent batched kernel, 0: one workgroup per item (default: Batch\'s)')
     args = ap.parse_args(argv)
     print(json.dumps(run_config4(args.steps, args.warmup, args.form, graphs=not args.no_graphs,
-                                 store_mac=args.store_mac)), flush=True)
+                                 store_mac=args.store_mac, persistent=args.persistent)), flush=True)
     return 0
 
 

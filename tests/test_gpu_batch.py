@@ -138,8 +138,9 @@ def test_batch_with_a_large_tunnel_equals_alone(gpu, persist):
 @pytest.mark.parametrize('cpt', [None, 2, 4])
 def test_batch_runs_every_tunnel_at_the_batch_cells_per_thread(gpu, cpt):
     """By default a batch runs every tunnel at the cells per thread tuned for
-    batches of its storage form (or at an explicit one), whatever the tunnels'
-    own launch shapes, which it leaves alone; results equal the tunnels alone."""
+    tunnels of its size class in batches of its storage form (or at an explicit
+    one), whatever the tunnels' own launch shapes, which it leaves alone; results
+    equal the tunnels alone."""
     from windtunnel import autotune
     from windtunnel._lib import WT_FLAG_FORCES
     from windtunnel.batch import Batch
@@ -147,9 +148,9 @@ def test_batch_runs_every_tunnel_at_the_batch_cells_per_thread(gpu, cpt):
     for t in mixed:
         t.shape = (2, 0)
     b = Batch(mixed, PARAMS, cpt=cpt)
-    want = cpt or autotune.batch_cpt('fp16')
-    assert b.info.max_cpt == want
-    assert b.info.total_tiles == sum(-(-t.n // (256 * want)) for t in mixed)
+    want = [cpt or autotune.batch_cpt('fp16', t.n) for t in mixed]
+    assert b.info.max_cpt == max(want)
+    assert b.info.total_tiles == sum(-(-t.n // (256 * w)) for t, w in zip(mixed, want))
     assert all(t.shape == (2, 0) for t in mixed)
     for _ in range(2):
         b.step(WT_FLAG_FORCES)

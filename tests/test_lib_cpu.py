@@ -313,9 +313,11 @@ def test_autotune_classes_follow_the_library_cache_policy(form):
 
 
 def test_autotune_batch_table():
-    """The committed tuning table holds one cells-per-thread value per storage form
-    for batches, chosen from the timed candidates; the tuning batch spans the
-    serving mix's sizes and the 48-tunnel batch's."""
+    """The committed tuning table holds, for batches, one cells-per-thread value
+    per storage form, chosen from the timed uniform candidates, and one per size
+    class found from it by coordinate descent (a class keeps a value only where
+    its timing beat the uniform one); a tunnel in a batch takes its class's value.
+    The tuning batch spans the serving mix's sizes and the 48-tunnel batch's."""
     import json
     from windtunnel import autotune
     with open(autotune.TUNING_PATH) as fh:
@@ -324,8 +326,19 @@ def test_autotune_batch_table():
         c = autotune.batch_cpt(form)
         assert c in autotune.BATCH_CPTS and c == table['batch_cpt'][form]
         ms = table['batch_timings'][form]
-        assert min(ms, key=ms.get) == str(c)
+        uniform = {k: v for k, v in ms.items() if k in {str(x) for x in autotune.BATCH_CPTS}}
+        assert min(uniform, key=uniform.get) == str(c)
+        per_class = table['batch_cpt_class'][form]
+        assert set(per_class) <= {name for name, _ in autotune.CLASSES}
+        for name, dims in autotune.CLASSES:
+            n = dims[0] * dims[1] * dims[2]
+            assert autotune.size_class(n, form) == name
+            assert autotune.batch_cpt(form, n) == per_class.get(name, c)
+        if per_class != {k: c for k in per_class}:
+            key = ','.join(f'{k}{v}' for k, v in sorted(per_class.items()))
+            assert ms[key] < ms[str(c)]
     assert autotune.batch_cpt('no-such-form') == 1
+    assert autotune.batch_cpt('no-such-form', 10 ** 6) == 1
     dims = autotune.batch_dims()
     assert len(dims) == autotune.BATCH_SERVING + autotune.BATCH_SMALL
     big = [d[0] * d[1] * d[2] for d in dims[:autotune.BATCH_SERVING]]

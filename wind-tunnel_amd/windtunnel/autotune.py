@@ -9,12 +9,15 @@ library uses for the tunnel's size) the tuner times every shape on a
 representative tunnel and stores the fastest in tuning.json beside this file;
 tunnels look their shape up by size class when they are created.
 
-Batches are tuned apart (tune_batch): one cells-per-thread value for every
-tunnel of a batch of that storage form, timed on a batch that spans the size
-classes. The batched kernel is compiled per largest cells-per-thread of its
-batch, and its one-cell variant holds more waves per SIMD, so a batch does not
-take its members' single-tunnel shapes (an FP16 batch of 60 serving-mix tunnels
-measured 7 % faster at one cell per thread than with the L class's two).
+Batches are tuned apart (tune_batch), on a batch that spans the size classes:
+first one cells-per-thread value for every tunnel of a batch of that storage
+form, then, from there, a value per size class (each class in turn tries every
+value while the others keep theirs, kept if it is faster), which a batch gives
+each tunnel by its class. The batched kernel is compiled per largest
+cells-per-thread of its batch, and its one-cell variant holds more waves per
+SIMD, so a batch does not take its members' single-tunnel shapes (an FP16 batch
+of 60 serving-mix tunnels measured 7 % faster at one cell per thread than with
+the L class's two).
 
     python -m windtunnel.autotune [--forms fp32,fp16,bf16,fp64] [--steps 40] [--batch-only]
 """
@@ -82,9 +85,16 @@ def shape_for(form_name, cells):
     return tuple(v) if v else DEFAULT
 
 
-def batch_cpt(form_name):
-    """Cells per thread of every tunnel in a batch of this storage form."""
-    return int(_load_file().get('batch_cpt', {}).get(form_name, 1))
+def batch_cpt(form_name, cells=None):
+    """Cells per thread of a tunnel of this many cells in a batch of this storage
+    form: the value tuned for its size class, else the form's batch value."""
+    t = _load_file()
+    per_class = t.get('batch_cpt_class', {}).get(form_name)
+    if cells is not None and per_class:
+        v = per_class.get(size_class(cells, form_name))
+        if v:
+            return int(v)
+    return int(t.get('batch_cpt', {}).get(form_name, 1))
 
 
 def tune(forms, steps=40):
@@ -143,15 +153,16 @@ def batch_dims(seed=3):
 
 
 def tune_batch(forms, steps=20):
-    """Time one batch per storage form (batch_dims(): tunnels of the sizes batches
-    hold, forces every step, no macroscopic stores) at each uniform
-    cells-per-thread value; returns {'batch_cpt': {form: best},
-    'batch_timings': {form: {cpt: ms}}}."""
+    """Tune the cells per thread of batched tunnels on one batch per storage form
+    (batch_dims(): tunnels of the sizes batches hold, forces every step, no
+    macroscopic stores): a uniform value first, then one per size class by
+    coordinate descent from it. Returns {'batch_cpt': {form: uniform best},
+    'batch_cpt_class': {form: {class: cpt}}, 'batch_timings': {form: {...: ms}}}."""
     import numpy as np
     import torch
     from .batch import Batch
     from .tunnel import Tunnel
-    out = {'batch_cpt': {}, 'batch_timings': {}}
+    out = {'batch_cpt': {}, 'batch_cpt_class': {}, 'batch_timings': {}}
     for form in forms:
         tunnels = []
         for dims in batch_dims():
@@ -160,10 +171,15 @@ def tune_batch(forms, steps=20):
             t.set_solid(_class_mask(dims))
             tunnels.append(t)
         params = [(0.05, 0.52, 0.16)] * len(tunnels)
-        res = {}
-        for rep in range(2):
-            for cpt in (BATCH_CPTS if rep == 0 else BATCH_CPTS[::-1]):
-                b = Batch(tunnels, params, cpt=cpt)
+        classes = [size_class(t.n, form) for t in tunnels]
+
+        def time_of(cpts):
+            # cpts: cells per thread per tunnel; median of two runs
+            runs = []
+            for _ in range(2):
+                for t, c in zip(tunnels, cpts):
+                    t.shape = (c, t.shape[1])
+                b = Batch(tunnels, params, own_shapes=True)
                 b.set_gates([False] * len(tunnels))
                 for _ in range(3):
                     b.step(1 | 4)
@@ -174,13 +190,34 @@ def tune_batch(forms, steps=20):
                     b.step(1 | 4)
                 e1.record()
                 torch.cuda.synchronize()
-                res.setdefault(str(cpt), []).append(e0.elapsed_time(e1) / steps)
+                runs.append(e0.elapsed_time(e1) / steps)
                 del b
-        ms = {k: float(np.median(v)) for k, v in res.items()}
+            return float(np.median(runs))
+
+        ms = {}
+        for cpt in BATCH_CPTS:
+            ms[str(cpt)] = time_of([cpt] * len(tunnels))
         best = min(BATCH_CPTS, key=lambda c: ms[str(c)])
+        per_class = {c: best for c in sorted(set(classes))}
+        cur = ms[str(best)]
+        for cl in sorted(set(classes)):
+            for cpt in BATCH_CPTS:
+                if cpt == per_class[cl]:
+                    continue
+                trial = dict(per_class, **{cl: cpt})
+                key = ','.join(f'{k}{v}' for k, v in sorted(trial.items()))
+                ms[key] = time_of([trial[c] for c in classes])
+                # keep only a clear gain: a class whose value raises the batch's largest
+                # cells per thread moves every tunnel to the slower kernel variant, which the
+                # tuning batch can hide within ~0.5 % (FP64: 0.56 % faster here, 1.6 % slower
+                # on the 48-tunnel batch)
+                if ms[key] < cur * 0.99:
+                    per_class, cur = trial, ms[key]
         out['batch_cpt'][form] = best
+        out['batch_cpt_class'][form] = per_class
         out['batch_timings'][form] = ms
-        print(json.dumps({'form': form, 'batch_tunnels': len(tunnels), 'best_cpt': best, 'ms': ms}), flush=True)
+        print(json.dumps({'form': form, 'batch_tunnels': len(tunnels), 'best_cpt': best, 'per_class': per_class,
+                          'ms': ms}), flush=True)
         del tunnels
         torch.cuda.empty_cache()
     return out
@@ -201,7 +238,7 @@ def main(argv=None):
     else:
         res = tune(forms, args.steps)
     b = tune_batch(forms)
-    for key in ('batch_cpt', 'batch_timings'):
+    for key in ('batch_cpt', 'batch_cpt_class', 'batch_timings'):
         res.setdefault(key, {}).update(b[key])
     with open(args.out, 'w') as fh:
         json.dump(res, fh, indent=1)

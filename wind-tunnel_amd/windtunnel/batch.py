@@ -29,8 +29,9 @@ class Batch:
         """tunnels: list of Tunnel (same storage form); params: list of (uin, tau, smag).
 
         Every tunnel runs at cpt cells per thread (default: the value tuned for
-        batches of this storage form, autotune.batch_cpt), unless own_shapes, when
-        each keeps the cells per thread of its own launch shape. persistent picks
+        tunnels of its size class in batches of this storage form,
+        autotune.batch_cpt), unless own_shapes, when each keeps the cells per
+        thread of its own launch shape. persistent picks
         the persistent kernel (default: PERSISTENT), run with persist_blocks
         workgroups if positive, else at the device's full occupancy."""
         if not tunnels:
@@ -40,8 +41,10 @@ class Batch:
             raise ValueError(f'a batch holds one storage form, got {sorted(forms)}')
         self.lib = _lib.load()
         self.tunnels = list(tunnels)
-        self.cpt = None if own_shapes else int(cpt if cpt is not None else
-                                                autotune.batch_cpt(_lib.FORM_NAMES[self.tunnels[0].form]))
+        form_name = _lib.FORM_NAMES[self.tunnels[0].form]
+        # cells per thread of each tunnel inside the batch (None: their own shapes)
+        self.cpts = None if own_shapes else [int(cpt) if cpt is not None else autotune.batch_cpt(form_name, t.n)
+                                            for t in self.tunnels]
         self.params = [tuple(float(v) for v in p) for p in params]
         self.persistent = PERSISTENT if persistent is None else bool(persistent)
         self.persist_blocks = int(persist_blocks)
@@ -65,9 +68,9 @@ class Batch:
         # each tunnel's tiles hold 256 * cpt cells inside the one launch: the batch's
         # cells per thread, or with own_shapes that of the tunnel's launch shape
         grids = (Grid * nt)(*[t.grid for t in self.tunnels])
-        if self.cpt is not None:
-            for g in grids:
-                g.cpt = self.cpt
+        if self.cpts is not None:
+            for g, c in zip(grids, self.cpts):
+                g.cpt = c
         prm = (Params * nt)(*[Params(*p) for p in self.params])
         cur = (ctypes.c_int32 * nt)(*[t.cur for t in self.tunnels])
         # the descriptor table is written into pinned host memory and copied in stream

@@ -1079,6 +1079,8 @@ template <int FORM> constexpr bool persist_claim_wave() { return FORM == WT_FP32
 // are in and its stores not yet out, so the claim's atomic neither holds the
 // loads back nor waits for the stores): the hook publishes the claim that came
 // back since the previous item's hook and sends the next one, once per item.
+// It fires on every item: thread 0 always steps the tile's first cell (a tile
+// starts below n), and both step paths call the hook outside any per-lane branch.
 struct ClaimHook {
   uint32_t* slot;
   uint32_t* pend;

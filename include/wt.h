@@ -122,7 +122,7 @@ int64_t wt_tile_count(int nx, int ny, int nz);
  * the bytes of its two population buffers: 1 (above 2 GiB) streams the
  * populations' c_x = 0 planes and the macroscopic field past the caches,
  * 2 (within 224 MiB) keeps everything cached, 0 otherwise (population stores
- * non-temporal). Batched steps always use 0. No counterpart in the reference;
+ * non-temporal). Batched steps use 1 for FP32, 0 for the other forms. No counterpart in the reference;
  * it names the kernel variant a measurement refers to. */
 int wt_cache_policy(int nx, int ny, int nz, int form);
 

@@ -877,10 +877,12 @@ __device__ __forceinline__ void slot_flush(double (*lds)[4][4], double* partials
 
 // ---- the tile loop ------------------------------------------------------------------
 // POL, the cache policy of a launch (chosen by cache_policy on the host):
-//   0  batches and mid-size tunnels: populations stored non-temporally, the rest cached;
-//   1  one tunnel with more than 2 GiB of populations: also the macroscopic field
-//      stored and the c_x = 0 planes loaded non-temporally (33.5M cells: session
-//      steps 0.9-1.05 % faster for FP32, 0.7-1.2 % for FP16; batches and smaller
+//   0  FP16/BF16/FP64 batches and mid-size tunnels: populations stored
+//      non-temporally, the rest cached;
+//   1  FP32 batches and one tunnel with more than 2 GiB of populations: also the
+//      macroscopic field stored and the c_x = 0 planes loaded non-temporally
+//      (33.5M cells: session steps 0.9-1.05 % faster for FP32, 0.7-1.2 % for FP16;
+//      the 48-tunnel FP32 batch 1.3-1.5 % faster, FP16 1 % slower; smaller single
 //      tunnels lose with it);
 //   2  one tunnel whose two population buffers fit in 224 MiB: everything cached,
 //      so the Infinity Cache carries a step's output into the next step.
@@ -968,6 +970,10 @@ __global__ __launch_bounds__(BLOCK) WT_STEP_ATTR void k_step_adv(StepDesc d, Adv
 // one-cell variant (64 registers, no spills; measured 1.6 % faster than its
 // natural 7), no constraint otherwise.
 template <int FORM, int MAXCPT> constexpr int batch_min_waves() { return Form<FORM>::dev && MAXCPT == 1 ? 8 : 1; }
+// Cache policy of the batched step (run_tile's POL): 1 for FP32 (the 48-tunnel
+// batch 1.3-1.5 % faster than with 0, the serving window the same), 0 otherwise
+// (FP16: 1 % slower with 1; the all-cached 2: FP32 13.6 %, FP16 1.7 % slower).
+template <int FORM> constexpr int batch_policy() { return FORM == WT_FP32 ? 1 : 0; }
 
 // gfx950's constant address space (scalar loads of memory no launch writes); the
 // host pass of this single-source file sees plain pointers
@@ -1016,12 +1022,12 @@ __device__ __forceinline__ void batch_item(const StepDesc* __restrict__ ds, cons
   const uint32_t gw = ((CWord)gates)[lo >> 2];
   const int fl = ((flags & WT_FLAG_NO_MAC) && ((gw >> (8 * (lo & 3))) & 0xffu)) ? (flags & ~WT_FLAG_NO_MAC) : flags;
   if constexpr (MAXCPT >= 4) {
-    if (d.cpt == 4) { run_tile<FORM, 4, 0, NBUF, H>(d, p, tile, fl, lbuf, hook); return; }
+    if (d.cpt == 4) { run_tile<FORM, 4, batch_policy<FORM>(), NBUF, H>(d, p, tile, fl, lbuf, hook); return; }
   }
   if constexpr (MAXCPT >= 2) {
-    if (d.cpt == 2) { run_tile<FORM, 2, 0, NBUF, H>(d, p, tile, fl, lbuf, hook); return; }
+    if (d.cpt == 2) { run_tile<FORM, 2, batch_policy<FORM>(), NBUF, H>(d, p, tile, fl, lbuf, hook); return; }
   }
-  run_tile<FORM, 1, 0, NBUF, H>(d, p, tile, fl, lbuf, hook);
+  run_tile<FORM, 1, batch_policy<FORM>(), NBUF, H>(d, p, tile, fl, lbuf, hook);
 }
 
 template <int FORM, int MAXCPT>

@@ -1432,10 +1432,11 @@ void launch_step_adv_f(const StepDesc& d, const AdvectArgs& a, Shape sh, int fla
   }
 }
 
-// The batched FP32 step runs best at 6 waves per SIMD (48 tunnels: 7 waves 1.9 %
-// slower, 5 waves 0.2 % slower); 22 KiB of LDS per workgroup holds it there
-// whatever its register count. FP16/BF16 take their 7 waves (65 registers).
-constexpr size_t FP32_BATCH_PAD_LDS = 22 * 1024;
+// The batched FP32 step (streaming cache policy) runs best at 5 waves per SIMD
+// (48 tunnels, same box: 6 waves 0.4-0.8 % slower, 7 waves 1.7 %, 4 waves 0.8 %);
+// 28 KiB of LDS per workgroup holds it there whatever its register count.
+// FP16/BF16 take their 8 waves (64 registers).
+constexpr size_t FP32_BATCH_PAD_LDS = 28 * 1024;
 template <int FORM> constexpr size_t batch_pad_lds() { return FORM == WT_FP32 ? FP32_BATCH_PAD_LDS : 0; }
 // Workgroups of a persistent launch: as many as the device holds at once (the
 // kernel's occupancy per CU times the CUs), found once per device and kernel.

@@ -1387,20 +1387,35 @@ struct Shape {
   int solo;   // the cache policy of the launch (run_tile's POL: 0, 1 or 2)
 };
 
-// Workgroups of the FP32 single-tunnel step kernels reserve LDS they do not use,
-// which holds them to 5 per CU (5 waves per SIMD, 160 KB of LDS per CU): at 80
-// VGPRs the hardware would run 6 waves, and at 33.5M cells 6 waves measured
-// 1.7 % slower per session step than 5 (FP16/BF16 run best at their own 7).
+// Workgroups of the single-tunnel step kernels reserve LDS they do not use,
+// which holds them to fewer waves than their registers allow (160 KB of LDS per
+// CU). FP32: under the streaming policy 1 (tunnels above 2 GiB of populations)
+// 4 waves per SIMD with 36 KiB (33.5M cells: 1.8-2.0 % faster than 5; 3 waves
+// 0.7 % faster than 5), otherwise 5 with 28 KiB (4.2M cells: 4 waves 1 % slower,
+// 0.5M cells 3.5 % slower; 6 waves 1.7 % slower at 33.5M cells). FP16/BF16:
+// under policy 1 6 waves with 22 KiB for the plain step (33.5M cells: 1.0-1.1 %
+// faster than their natural 7, 5 waves 1 % slower), otherwise their 7, and their
+// 7 for the step fused with the advection (the reserve holds the advection's
+// workgroups back too: FP16 sessions 0.6 % slower with it).
 constexpr size_t FP32_STEP_PAD_LDS = 28 * 1024;
-template <int FORM> constexpr size_t step_pad_lds() { return FORM == WT_FP32 ? FP32_STEP_PAD_LDS : 0; }
+constexpr size_t FP32_STREAM_STEP_PAD_LDS = 36 * 1024;
+constexpr size_t HALF_STREAM_STEP_PAD_LDS = 22 * 1024;
+template <int FORM, int POL> constexpr size_t step_pad_lds() {
+  if (FORM == WT_FP32) return POL == 1 ? FP32_STREAM_STEP_PAD_LDS : FP32_STEP_PAD_LDS;
+  if (Form<FORM>::dev) return POL == 1 ? HALF_STREAM_STEP_PAD_LDS : 0;
+  return 0;
+}
+template <int FORM, int POL> constexpr size_t step_adv_pad_lds() {
+  return FORM == WT_FP32 ? step_pad_lds<FORM, POL>() : 0;
+}
 
 template <int FORM, int CPT>
 static void launch_step_t(const StepDesc& d, int parity, int remap, int flags, int solo, hipStream_t s) {
   const uint32_t ntile = (d.n + BLOCK * CPT - 1) / (BLOCK * CPT);
   switch (solo) {
-    case 1: hipLaunchKernelGGL((k_step<FORM, CPT, 1>), dim3(ntile), dim3(BLOCK), step_pad_lds<FORM>(), s, d, parity, remap, flags); break;
-    case 2: hipLaunchKernelGGL((k_step<FORM, CPT, 2>), dim3(ntile), dim3(BLOCK), step_pad_lds<FORM>(), s, d, parity, remap, flags); break;
-    default: hipLaunchKernelGGL((k_step<FORM, CPT, 0>), dim3(ntile), dim3(BLOCK), step_pad_lds<FORM>(), s, d, parity, remap, flags); break;
+    case 1: hipLaunchKernelGGL((k_step<FORM, CPT, 1>), dim3(ntile), dim3(BLOCK), (step_pad_lds<FORM, 1>()), s, d, parity, remap, flags); break;
+    case 2: hipLaunchKernelGGL((k_step<FORM, CPT, 2>), dim3(ntile), dim3(BLOCK), (step_pad_lds<FORM, 2>()), s, d, parity, remap, flags); break;
+    default: hipLaunchKernelGGL((k_step<FORM, CPT, 0>), dim3(ntile), dim3(BLOCK), (step_pad_lds<FORM, 0>()), s, d, parity, remap, flags); break;
   }
 }
 
@@ -1418,9 +1433,9 @@ static void launch_step_adv_t(const StepDesc& d, const AdvectArgs& a, int remap,
   const uint32_t ntile = (d.n + BLOCK * CPT - 1) / (BLOCK * CPT);
   const uint32_t nadv = (a.num_p + BLOCK - 1) / BLOCK;
   switch (solo) {
-    case 1: hipLaunchKernelGGL((k_step_adv<FORM, CPT, 1>), dim3(ntile + nadv), dim3(BLOCK), step_pad_lds<FORM>(), s, d, a, 0, remap, flags, ntile); break;
-    case 2: hipLaunchKernelGGL((k_step_adv<FORM, CPT, 2>), dim3(ntile + nadv), dim3(BLOCK), step_pad_lds<FORM>(), s, d, a, 0, remap, flags, ntile); break;
-    default: hipLaunchKernelGGL((k_step_adv<FORM, CPT, 0>), dim3(ntile + nadv), dim3(BLOCK), step_pad_lds<FORM>(), s, d, a, 0, remap, flags, ntile); break;
+    case 1: hipLaunchKernelGGL((k_step_adv<FORM, CPT, 1>), dim3(ntile + nadv), dim3(BLOCK), (step_adv_pad_lds<FORM, 1>()), s, d, a, 0, remap, flags, ntile); break;
+    case 2: hipLaunchKernelGGL((k_step_adv<FORM, CPT, 2>), dim3(ntile + nadv), dim3(BLOCK), (step_adv_pad_lds<FORM, 2>()), s, d, a, 0, remap, flags, ntile); break;
+    default: hipLaunchKernelGGL((k_step_adv<FORM, CPT, 0>), dim3(ntile + nadv), dim3(BLOCK), (step_adv_pad_lds<FORM, 0>()), s, d, a, 0, remap, flags, ntile); break;
   }
 }
 template <int FORM>

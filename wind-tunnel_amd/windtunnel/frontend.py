@@ -137,7 +137,7 @@ class MultiEngine(threading.Thread):
         return need + (self.used - s.bytes) <= self.budget
 
     def hello(self, s):
-        """app.py:78-82, `fits` judged against the memory this session may still use."""
+        """app.py:78-82, `fits` checked against the memory this session may still use."""
         res = [{'name': k, 'dims': list(v), 'fits': self.fits(v, s.storage, s)} for k, v in self.resolutions.items()]
         high = self.resolutions.get('high')
         default = 'high' if high and self.fits(high, s.storage, s) else 'test'

@@ -48,9 +48,6 @@
 namespace wt {
 
 constexpr int BLOCK = 256;       // threads per workgroup (4 wave64)
-#ifndef WT_STEP_ATTR
-#define WT_STEP_ATTR
-#endif
 // The thread index as the step path reads it. The persistent batched step's unit
 // (wt_step_persist.hip) reads it through an empty asm at every use, so that the
 // compiler cannot hoist what depends on it out of the item loop.
@@ -81,24 +78,16 @@ template <typename T> __device__ __forceinline__ T maxv(T a, T b) { return (a < 
 // lines one wave reads whole, 0.9 % (FP32) and 0.5 % (FP16) faster per session
 // step. Storing the macroscopic field non-temporally made session steps (step,
 // particle advection and slices, which read the field) 0.8 % faster for FP16 and
-// left FP32 the same. Batched steps lose with both (0.35 and 0.8 %), and so do
-// single tunnels whose state partly fits the Infinity Cache, so only launches of
-// one tunnel with more than 2 GiB of populations use them (run_tile's POL 1,
-// solo_policy), on the raw-buffer paths; the 64-bit global-address path loads
-// every plane cached. A tunnel whose populations fit in 224 MiB stores them cached
-// too (POL 2).
-#ifndef WT_LOAD_NT
-#define WT_LOAD_NT 0
-#endif
-#ifndef WT_STORE_NT
-#define WT_STORE_NT 1
-#endif
-template <typename E> __device__ __forceinline__ E ld_stream(const E* p) {
-  if constexpr (WT_LOAD_NT) return __builtin_nontemporal_load(p);
-  else return *p;
-}
+// left FP32 the same. Single tunnels whose state partly fits the Infinity Cache
+// lose with both, so of the single-tunnel launches only those with more than
+// 2 GiB of populations use them (run_tile's POL 1, solo_policy); batched steps
+// use them for FP32 (0.610 against 0.614-0.634 ms for the 48-tunnel batch) and
+// not for the 2-byte forms (1 % slower), batch_policy. Both apply on the
+// raw-buffer paths; the 64-bit global-address path loads every plane cached. A
+// tunnel whose populations fit in 224 MiB stores them cached too (POL 2).
+template <typename E> __device__ __forceinline__ E ld_stream(const E* p) { return *p; }
 template <bool NT, typename E> __device__ __forceinline__ void st_stream(E* p, E v) {
-  if constexpr (NT && WT_STORE_NT) __builtin_nontemporal_store(v, p);
+  if constexpr (NT) __builtin_nontemporal_store(v, p);
   else *p = v;
 }
 
@@ -561,14 +550,8 @@ constexpr int BUF_GFX9_WORD3 = 0x00020000;  // raw buffer, gfx9-family resource 
 // loads in this order measured 0.5-0.7 % faster than direction order.
 constexpr int LOAD_ORDER[Q] = {15, 5, 11, 14, 18, 3, 7, 10, 0, 1, 2, 4, 8, 9, 17, 6, 12, 13, 16};
 // cache-policy word of the raw-buffer loads and stores (gfx950: bit 0 sc0, bit 1 nt, bit 4 sc1)
-#ifndef WT_CPOL_LD
-#define WT_CPOL_LD (WT_LOAD_NT ? 2 : 0)
-#endif
-#ifndef WT_CPOL_ST
-#define WT_CPOL_ST (WT_STORE_NT ? 2 : 0)
-#endif
-constexpr int CPOL_LD = WT_CPOL_LD;
-constexpr int CPOL_ST = WT_CPOL_ST;
+constexpr int CPOL_LD = 0;      // population loads, cached
+constexpr int CPOL_ST = 2;      // non-temporal population stores
 constexpr int CPOL_LD_NT = 2;   // loads of the c_x = 0 planes
 
 template <typename E> struct RawIO;
@@ -930,7 +913,7 @@ __device__ __forceinline__ uint32_t xcd_tile(uint32_t b, uint32_t nblocks, int r
 }
 
 template <int FORM, int CPT, int POL>
-__global__ __launch_bounds__(BLOCK) WT_STEP_ATTR void k_step(StepDesc d, int parity, int remap, int flags) {
+__global__ __launch_bounds__(BLOCK) void k_step(StepDesc d, int parity, int remap, int flags) {
   const uint32_t tile = xcd_tile(blockIdx.x, gridDim.x, remap);
   run_tile<FORM, CPT, POL>(d, parity, tile, flags);
 }
@@ -943,7 +926,7 @@ __global__ __launch_bounds__(BLOCK) WT_STEP_ATTR void k_step(StepDesc d, int par
 // particles: advection first costs ~7 us over the plain step, spread evenly
 // through the tiles ~14 us, after the tiles ~19 us, as much as its own launch.)
 template <int FORM, int CPT, int POL>
-__global__ __launch_bounds__(BLOCK) WT_STEP_ATTR void k_step_adv(StepDesc d, AdvectArgs a, int parity, int remap,
+__global__ __launch_bounds__(BLOCK) void k_step_adv(StepDesc d, AdvectArgs a, int parity, int remap,
                                                                  int flags, uint32_t step_blocks) {
   const uint32_t nadv = gridDim.x - step_blocks;
   if (blockIdx.x >= nadv) {

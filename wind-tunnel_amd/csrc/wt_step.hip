@@ -944,11 +944,13 @@ __global__ __launch_bounds__(BLOCK) void k_step_adv(StepDesc d, AdvectArgs a, in
 // MAXCPT, the batch's largest: the variant without the 2- and 4-cell paths
 // needs 64-65 registers instead of 80; FP16 on 48 tunnels 10.7 % faster). With
 // `backwards` the items are dispatched from the last to the first. Each cell is
-// computed exactly as by k_step. (A persistent form, a grid of one occupancy
-// striding over the items, measured 16 % slower on 48 uneven tunnels: the loop
-// raised the kernel to 96 registers, 5 waves per SIMD instead of 7; on the
-// one-cell variant it spilled 21-74 scalar registers, 106-115 VGPRs for FP32,
-// 25 VGPRs to scratch for FP16.)
+// computed exactly as by k_step. The persistent form, k_step_batch_persist
+// below, runs 1.7-3.9 % (FP32) and 8-10 % (FP16) slower on 48 tunnels and is
+// not the default. Its first version, a grid of one occupancy striding over the
+// items in this unit, measured 16 % slower: the loop raised the kernel to 96
+// registers, 5 waves per SIMD instead of 7; on the one-cell variant it spilled
+// 21-74 scalar registers, 106-115 VGPRs for FP32, 25 VGPRs to scratch for FP16.
+// Those register measures are why it now has a unit of its own.
 // Waves per SIMD the compiler fits the batched step to: 8 for the FP16/BF16
 // one-cell variant (64 registers, no spills; measured 1.6 % faster than its
 // natural 7), no constraint otherwise.

@@ -26,8 +26,8 @@
 //
 // k_step_adv launches a step together with the particle advection of the
 // previous step (advect_kernel, lbm.py:233-263, shared code in wt_particles.h):
-// its first workgroups advect from the macroscopic buffer that step wrote, the
-// rest run the step's tiles into the other buffer.
+// workgroups spread through its grid advect from the macroscopic buffer that
+// step wrote, the rest run the step's tiles into the other buffer.
 //
 // The whole library is compiled with -ffp-contract=off: each operation is
 // rounded once, in the order the reference's generated source uses, so the
@@ -920,19 +920,37 @@ __global__ __launch_bounds__(BLOCK) void k_step(StepDesc d, int parity, int rema
 
 // One lattice step plus the particle advection of the previous step from its
 // macroscopic field a.mac (a buffer the step does not write), in one launch.
-// The advection takes the first gridDim.x - step_blocks workgroups: its
-// latency-bound gathers then overlap the ramp-up of the HBM-bound tiles, which
-// follow in the tile order of k_step. (Measured at 33.5M cells and 2^20
-// particles: advection first costs ~7 us over the plain step, spread evenly
-// through the tiles ~14 us, after the tiles ~19 us, as much as its own launch.)
+// The nadv = gridDim.x - step_blocks advection workgroups are spread through
+// the grid: with s = gridDim.x / nadv, workgroup q s advects particle block q
+// (q < nadv) and every other workgroup runs the next tile in the tile order of
+// k_step, so the latency-bound gathers run beside the HBM-bound tiles
+// throughout. Measured with the FP16 session step at 33.5M cells and 2^20
+// particles (two boxes, 3 and 4 alternating runs): the fused kernel's median
+// 0.536-0.542 ms spread, 0.539-0.545 ms with the advection taking the first
+// workgroups, 0.547-0.549 ms after the tiles. (An earlier spread over the
+// kernels of the time had measured slower than advection first.)
+// Every workgroup maps to exactly one tile or particle block.
 template <int FORM, int CPT, int POL>
 __global__ __launch_bounds__(BLOCK) void k_step_adv(StepDesc d, AdvectArgs a, int parity, int remap,
                                                                  int flags, uint32_t step_blocks) {
   const uint32_t nadv = gridDim.x - step_blocks;
-  if (blockIdx.x >= nadv) {
-    run_tile<FORM, CPT, POL>(d, parity, xcd_tile(blockIdx.x - nadv, step_blocks, remap), flags);
+  const uint32_t b = blockIdx.x;
+  bool adv = false;
+  uint32_t k = b;  // tile index, or particle block when adv
+  if (nadv > 0) {
+    const uint32_t s = gridDim.x / nadv;
+    if (b < nadv * s) {
+      const uint32_t q = b / s, r = b - q * s;
+      adv = r == 0;
+      k = adv ? q : q * (s - 1) + r - 1;
+    } else {
+      k = nadv * (s - 1) + (b - nadv * s);
+    }
+  }
+  if (!adv) {
+    run_tile<FORM, CPT, POL>(d, parity, xcd_tile(k, step_blocks, remap), flags);
   } else {
-    const uint32_t j = blockIdx.x * BLOCK + threadIdx.x;
+    const uint32_t j = k * BLOCK + threadIdx.x;
     if (j < a.num_p) advect_particle(a, j);
   }
 }

@@ -65,6 +65,22 @@ def test_fused_advection_equals_separate_launches(gpu, form):
     assert np.isfinite(a['pos']).all() and (a['spd'] > 0).all()
 
 
+@pytest.mark.parametrize('form', ['fp32', 'fp16'])
+def test_fused_advection_with_more_particle_blocks_than_tiles(gpu, form):
+    """A 24x16x12 tunnel (18 tiles of 256 cells) with 20011 particles (79
+    workgroups of them): the fused launch's grid holds more advection workgroups
+    than tiles, so nothing is spread between them (its first 79 workgroups
+    advect and the 18 tiles follow). Results equal the separate launches bit
+    for bit."""
+    from windtunnel import geometry
+    dims = (24, 16, 12)
+    mask = geometry.sphere(dims, (8, 8, 6), 3)[0]
+    a = _run(7, dims, mask, 13, fuse=True, form=form, peek=(4,))
+    b = _run(7, dims, mask, 13, fuse=False, form=form)
+    _same(a, b)
+    assert np.isfinite(a['pos']).all()
+
+
 def test_step_advect_rejects_aliased_mac(gpu):
     """The advected field must not be the buffer the same launch writes."""
     from windtunnel.session import Session

@@ -929,7 +929,11 @@ __global__ __launch_bounds__(BLOCK) void k_step(StepDesc d, int parity, int rema
 // 0.536-0.542 ms spread, 0.539-0.545 ms with the advection taking the first
 // workgroups, 0.547-0.549 ms after the tiles. (An earlier spread over the
 // kernels of the time had measured slower than advection first.)
-// Every workgroup maps to exactly one tile or particle block.
+// Every workgroup maps to exactly one tile or particle block. The tile order's
+// XCD bit (xcd_tile, bit 0) sees the tile's rank k among the step workgroups,
+// not the workgroup index, so in this grid a tile's XCD follows the spread
+// rather than that order: only the speed of the shapes that set the bit
+// (tunnels below 200k cells) can change, never a result.
 template <int FORM, int CPT, int POL>
 __global__ __launch_bounds__(BLOCK) void k_step_adv(StepDesc d, AdvectArgs a, int parity, int remap,
                                                                  int flags, uint32_t step_blocks) {
